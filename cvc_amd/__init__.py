@@ -1,0 +1,310 @@
+"""Python binding of ``libcvchip.so`` (ctypes over the C ABI in include/cvchip.h).
+
+The HIP library is the only compute path: importing this module loads
+``cvc_amd/libcvchip.so`` and raises if it is missing.  Arrays use the
+reference's host layouts: spinor ``float64[V, 24]`` (or ``[n, V, 24]`` for n
+right-hand sides), gauge ``float64[V, 4, 18]``, conn ``float64[V, 16, 2]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcvchip.so")
+
+PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
+SOLVE_LAGGED, SOLVE_VERBOSE = 1, 2
+ERR_NULL, ERR_NOCONV = -2, -3
+
+# every entry point declared in include/cvchip.h
+EXPORTS = [
+    "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
+    "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
+    "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
+    "cvchip_field_destroy", "cvchip_field_upload", "cvchip_field_download", "cvchip_field_copy",
+    "cvchip_field_dslash",
+    "cvchip_field_dslash_timed", "cvchip_field_invert", "cvchip_field_invert_her", "cvchip_cvc_contract",
+    "cvchip_field_cvc_contract", "cvchip_wi", "cvchip_pi_q", "cvchip_cvc_source", "cvchip_gauge_random",
+]
+
+
+class CvchipError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"libcvchip error {rc}: {msg}")
+        self.rc = rc
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("reliable_updates", C.c_int),
+                ("dslash_calls", C.c_int), ("true_res2_max", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libcvchip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not found: build it with `python -m cvc_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.cvchip_last_error.restype = C.c_char_p
+        L.cvchip_ctx_create.argtypes = [C.POINTER(_vp), C.POINTER(C.c_int), _dp, C.c_double, C.c_double]
+        L.cvchip_ctx_destroy.argtypes = [_vp]
+        L.cvchip_ctx_set_params.argtypes = [_vp, C.c_double, C.c_double]
+        L.cvchip_ctx_set_link_recon.argtypes = [_vp, C.c_int]
+        L.cvchip_set_gauge.argtypes = [_vp, _dp]
+        L.cvchip_Q_phi.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int]
+        L.cvchip_scalar_product_co.argtypes = [_vp, _dp, _dp, _dp]
+        L.cvchip_scalar_product_re.argtypes = [_vp, _dp, _dp, _dp]
+        L.cvchip_invert_Qtm.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, C.c_uint, C.POINTER(Stats)]
+        L.cvchip_invert_Qtm_her.argtypes = [_vp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                            C.c_double, C.c_uint, C.POINTER(Stats)]
+        L.cvchip_field_create.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(_vp)]
+        L.cvchip_field_destroy.argtypes = [_vp]
+        L.cvchip_field_upload.argtypes = [_vp, _vp, _dp, C.c_int, C.c_int]
+        L.cvchip_field_download.argtypes = [_vp, _vp, _dp, C.c_int, C.c_int]
+        L.cvchip_field_copy.argtypes = [_vp, _vp, _vp]
+        L.cvchip_field_dslash.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int]
+        L.cvchip_field_dslash_timed.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int, _dp]
+        L.cvchip_field_invert.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_double,
+                                          C.c_uint, C.POINTER(Stats)]
+        L.cvchip_field_invert_her.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_double,
+                                              C.c_uint, C.POINTER(Stats)]
+        L.cvchip_cvc_contract.argtypes = [_vp, _dp, _dp, C.POINTER(_dp), C.POINTER(C.c_int)]
+        L.cvchip_field_cvc_contract.argtypes = [_vp, _dp, _dp, _vp, _vp, C.POINTER(C.c_int)]
+        L.cvchip_wi.argtypes = [_vp, _dp, _dp]
+        L.cvchip_pi_q.argtypes = [_vp, _dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.cvchip_cvc_source.argtypes = [_vp, _dp, _dp, C.POINTER(C.c_int), C.c_double, C.c_int, C.c_int,
+                                        C.c_double, C.POINTER(Stats)]
+        L.cvchip_gauge_random.argtypes = [C.POINTER(C.c_int), C.c_uint64, C.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().cvchip_last_error().decode(errors="replace")
+
+
+def _check(rc):
+    if rc < 0:
+        raise CvchipError(rc, last_error())
+    return rc
+
+
+def _ptr(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def gauge_random(dims, seed, eps=0.3) -> np.ndarray:
+    """U_mu(x) = exp(i eps H) with Philox-seeded Gaussian H (host, deterministic)."""
+    dims = [int(d) for d in dims]
+    V = int(np.prod(dims))
+    g = np.empty((V, 4, 18))
+    _check(lib().cvchip_gauge_random((C.c_int * 4)(*dims), C.c_uint64(int(seed)), float(eps), _ptr(g)))
+    return g
+
+
+def point_source(V, site, spin, colour) -> np.ndarray:
+    """Unit source at one site, spin-colour component (cvc_exact2_xspace.c:521-524)."""
+    b = np.zeros((V, 24))
+    b[site, 2 * (3 * spin + colour)] = 1.0
+    return b
+
+
+class Field:
+    """nrhs spinor fields resident in HBM (library layout)."""
+
+    def __init__(self, ctx: "Context", nrhs: int, prec: int = PREC_F64):
+        self.ctx, self.nrhs, self.prec = ctx, int(nrhs), int(prec)
+        h = _vp()
+        _check(lib().cvchip_field_create(ctx._h, self.nrhs, self.prec, C.byref(h)))
+        self._h = h
+
+    def upload(self, host: np.ndarray, rhs0: int = 0):
+        host = np.ascontiguousarray(host, dtype=np.float64).reshape(-1, self.ctx.V, 24)
+        _check(lib().cvchip_field_upload(self.ctx._h, self._h, _ptr(host), int(rhs0), host.shape[0]))
+
+    def download(self, rhs0: int = 0, n: int | None = None) -> np.ndarray:
+        n = self.nrhs - rhs0 if n is None else n
+        out = np.empty((n, self.ctx.V, 24))
+        _check(lib().cvchip_field_download(self.ctx._h, self._h, _ptr(out), int(rhs0), int(n)))
+        return out
+
+    def copy_from(self, src: "Field"):
+        _check(lib().cvchip_field_copy(self.ctx._h, self._h, src._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cvchip_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One lattice on the current GPU: geometry, kappa, mu, boundary phases, gauge."""
+
+    def __init__(self, dims, kappa, mu, bc=(1.0, 0.0, 0.0, 0.0), gauge=None):
+        self.dims = tuple(int(d) for d in dims)
+        self.V = int(np.prod(self.dims))
+        self.kappa, self.mu = float(kappa), float(mu)
+        h = _vp()
+        _check(lib().cvchip_ctx_create(C.byref(h), (C.c_int * 4)(*self.dims), (C.c_double * 4)(*bc),
+                                       self.kappa, self.mu))
+        self._h = h
+        if gauge is not None:
+            self.set_gauge(gauge)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cvchip_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, kappa, mu):
+        self.kappa, self.mu = float(kappa), float(mu)
+        _check(lib().cvchip_ctx_set_params(self._h, self.kappa, self.mu))
+
+    def set_link_recon(self, recon: int):
+        _check(lib().cvchip_ctx_set_link_recon(self._h, int(recon)))
+
+    def set_gauge(self, gauge: np.ndarray):
+        g = np.ascontiguousarray(gauge, dtype=np.float64)
+        assert g.size == self.V * 72
+        _check(lib().cvchip_set_gauge(self._h, _ptr(g)))
+
+    # -- host-array drop-ins ---------------------------------------------------
+    def Q_phi(self, phi, mutm=None, gamma5_out=False):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.empty((self.V, 24))
+        _check(lib().cvchip_Q_phi(self._h, _ptr(out), _ptr(phi), self.mu if mutm is None else float(mutm),
+                                  int(bool(gamma5_out))))
+        return out
+
+    def scalar_product_co(self, a, b):
+        w = np.zeros(2)
+        _check(lib().cvchip_scalar_product_co(self._h, _ptr(w), _ptr(np.ascontiguousarray(a)),
+                                              _ptr(np.ascontiguousarray(b))))
+        return complex(w[0], w[1])
+
+    def scalar_product_re(self, a, b):
+        w = np.zeros(1)
+        _check(lib().cvchip_scalar_product_re(self._h, _ptr(w), _ptr(np.ascontiguousarray(a)),
+                                              _ptr(np.ascontiguousarray(b))))
+        return float(w[0])
+
+    def invert_Qtm(self, b, sp, niter_max=1000, x0=None, prec=PREC_F64, delta=0.1, lagged=False,
+                   verbose=False, check=True):
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, self.V, 24)
+        n = b.shape[0]
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, order="C").reshape(b.shape)
+        st = Stats()
+        flags = (SOLVE_LAGGED if lagged else 0) | (SOLVE_VERBOSE if verbose else 0)
+        rc = lib().cvchip_invert_Qtm(self._h, _ptr(x), _ptr(b), n, float(sp), int(niter_max), int(prec),
+                                     float(delta), flags, C.byref(st))
+        if check and rc < 0:
+            raise CvchipError(rc, last_error())
+        return rc, x, st.as_dict()
+
+    def invert_Qtm_her(self, b, sp, niter_max=1000, x0=None, prec=PREC_F64, delta=0.1, check=True):
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, self.V, 24)
+        n = b.shape[0]
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, order="C").reshape(b.shape)
+        st = Stats()
+        rc = lib().cvchip_invert_Qtm_her(self._h, _ptr(x), _ptr(b), n, float(sp), int(niter_max), int(prec),
+                                         float(delta), 0, C.byref(st))
+        if check and rc < 0:
+            raise CvchipError(rc, last_error())
+        return rc, x, st.as_dict()
+
+    # -- device-resident fields ------------------------------------------------
+    def field(self, nrhs, prec=PREC_F64) -> Field:
+        return Field(self, nrhs, prec)
+
+    def dslash(self, out: Field, inp: Field, mutm=None, gamma5_out=False):
+        _check(lib().cvchip_field_dslash(self._h, out._h, inp._h, self.mu if mutm is None else float(mutm),
+                                         int(bool(gamma5_out))))
+
+    def dslash_timed(self, out: Field, inp: Field, reps: int, mutm=None) -> float:
+        ms = np.zeros(1)
+        _check(lib().cvchip_field_dslash_timed(self._h, out._h, inp._h, self.mu if mutm is None else float(mutm),
+                                               int(reps), _ptr(ms)))
+        return float(ms[0])
+
+    def invert(self, x: Field, b: Field, sp, niter_max=1000, prec=PREC_F64, delta=0.1, lagged=False,
+               check=True):
+        st = Stats()
+        rc = lib().cvchip_field_invert(self._h, x._h, b._h, float(sp), int(niter_max), int(prec), float(delta),
+                                       SOLVE_LAGGED if lagged else 0, C.byref(st))
+        if check and rc < 0:
+            raise CvchipError(rc, last_error())
+        return rc, st.as_dict()
+
+    def invert_her(self, x: Field, b: Field, sp, niter_max=1000, prec=PREC_F64, delta=0.1, check=True):
+        st = Stats()
+        rc = lib().cvchip_field_invert_her(self._h, x._h, b._h, float(sp), int(niter_max), int(prec),
+                                           float(delta), 0, C.byref(st))
+        if check and rc < 0:
+            raise CvchipError(rc, last_error())
+        return rc, st.as_dict()
+
+    # -- observables -----------------------------------------------------------
+    def cvc_contract(self, props, src):
+        assert len(props) == 120
+        arrs = [np.ascontiguousarray(p, dtype=np.float64) for p in props]
+        ptrs = (_dp * 120)(*[_ptr(a) for a in arrs])
+        conn = np.empty((self.V, 16, 2))
+        contact = np.zeros(8)
+        _check(lib().cvchip_cvc_contract(self._h, _ptr(conn), _ptr(contact), ptrs, (C.c_int * 4)(*src)))
+        return conn, contact[0::2] + 1j * contact[1::2]
+
+    def field_cvc_contract(self, up: Field, dn: Field, src):
+        conn = np.empty((self.V, 16, 2))
+        contact = np.zeros(8)
+        _check(lib().cvchip_field_cvc_contract(self._h, _ptr(conn), _ptr(contact), up._h, dn._h,
+                                               (C.c_int * 4)(*src)))
+        return conn, contact[0::2] + 1j * contact[1::2]
+
+    def wi(self, conn):
+        out = np.empty((self.V, 4, 2))
+        _check(lib().cvchip_wi(self._h, _ptr(out), _ptr(np.ascontiguousarray(conn))))
+        return out
+
+    def pi_q(self, conn, contact, src):
+        ct = np.zeros(8)
+        ct[0::2] = np.real(contact)
+        ct[1::2] = np.imag(contact)
+        out = np.empty((self.V, 16, 2))
+        _check(lib().cvchip_pi_q(self._h, _ptr(out), _ptr(np.ascontiguousarray(conn)), _ptr(ct),
+                                 (C.c_int * 4)(*src)))
+        return out
+
+    def cvc_source(self, src, sp, niter_max=1000, prec=PREC_F64, delta=0.1):
+        conn = np.empty((self.V, 16, 2))
+        contact = np.zeros(8)
+        st = Stats()
+        _check(lib().cvchip_cvc_source(self._h, _ptr(conn), _ptr(contact), (C.c_int * 4)(*src), float(sp),
+                                       int(niter_max), int(prec), float(delta), C.byref(st)))
+        return conn, contact[0::2] + 1j * contact[1::2], st.as_dict()

@@ -1,0 +1,322 @@
+// Context, device memory and layout conversion of libcvchip.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "cvchip_internal.h"
+
+namespace cvchip {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  if (e == hipErrorOutOfMemory) return CVCHIP_ERR_OOM;
+  return CVCHIP_ERR_HIP;
+}
+
+int ensure_stage(cvchip_ctx *c, size_t bytes) {
+  if (c->stage_bytes >= bytes) return 0;
+  if (c->stage) CVC_HIP(hipFree(c->stage));
+  c->stage = nullptr;
+  c->stage_bytes = 0;
+  CVC_HIP(hipMalloc(&c->stage, bytes));
+  c->stage_bytes = bytes;
+  return 0;
+}
+
+int ensure_red(cvchip_ctx *c, size_t bytes) {
+  if (c->red.bytes >= bytes) return 0;
+  if (c->red.ptr) CVC_HIP(hipFree(c->red.ptr));
+  c->red.ptr = nullptr;
+  c->red.bytes = 0;
+  CVC_HIP(hipMalloc(&c->red.ptr, bytes));
+  c->red.bytes = bytes;
+  return 0;
+}
+
+static size_t elem_bytes(int prec) {
+  return prec == CVCHIP_PREC_F64 ? sizeof(double2) : prec == CVCHIP_PREC_F32 ? sizeof(float2) : sizeof(__half2);
+}
+
+int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec) {
+  if (nrhs <= 0 || nrhs > 256) {
+    set_error("nrhs %d outside 1..256", nrhs);
+    return CVCHIP_ERR_ARG;
+  }
+  if (prec < CVCHIP_PREC_F64 || prec > CVCHIP_PREC_F16) {
+    set_error("unknown precision %d", prec);
+    return CVCHIP_ERR_ARG;
+  }
+  f.nrhs = nrhs;
+  f.prec = prec;
+  f.V = c->lat.V;
+  f.bytes = (size_t)12 * f.V * nrhs * elem_bytes(prec);
+  CVC_HIP(hipMalloc(&f.data, f.bytes));
+  if (prec == CVCHIP_PREC_F16) CVC_HIP(hipMalloc(&f.norm, sizeof(float) * f.V * nrhs));
+  return 0;
+}
+
+void field_free(Field &f) {
+  if (f.data) (void)hipFree(f.data);
+  if (f.norm) (void)hipFree(f.norm);
+  f.data = nullptr;
+  f.norm = nullptr;
+}
+
+// ------------------------------------------------------------- kernels
+
+// AoS staging double[n][V][24] -> field rhs rhs0..rhs0+n-1
+template <typename R>
+__global__ void k_aos_to_field(cplx<R> *__restrict__ f, const double *__restrict__ aos, int64_t V,
+                               int N, int rhs0, int n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // over n*V*12, comp fastest
+  if (i >= (int64_t)n * V * 12) return;
+  int c = (int)(i % 12);
+  int64_t sv = i / 12;
+  int64_t s = sv % V;
+  int r = (int)(sv / V);
+  const double *p = aos + ((int64_t)r * V + s) * 24 + 2 * c;
+  f[((int64_t)c * V + s) * N + rhs0 + r] = mk<R>((R)p[0], (R)p[1]);
+}
+
+template <typename R>
+__global__ void k_field_to_aos(double *__restrict__ aos, const cplx<R> *__restrict__ f, int64_t V,
+                               int N, int rhs0, int n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * V * 12) return;
+  int c = (int)(i % 12);
+  int64_t sv = i / 12;
+  int64_t s = sv % V;
+  int r = (int)(sv / V);
+  cplx<R> v = f[((int64_t)c * V + s) * N + rhs0 + r];
+  double *p = aos + ((int64_t)r * V + s) * 24 + 2 * c;
+  p[0] = (double)v.x;
+  p[1] = (double)v.y;
+}
+
+// gauge AoS double[V][4][18] -> recon-18 FP64 [4][9][V], recon-12 FP64 [4][6][V],
+// recon-12 FP32 [4][6][V]
+__global__ void k_gauge_convert(double2 *__restrict__ g18, double2 *__restrict__ g12,
+                                float2 *__restrict__ g12f, const double *__restrict__ aos, int64_t V) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // over V*4*9
+  if (i >= V * 36) return;
+  int k = (int)(i % 9);
+  int mu = (int)((i / 9) % 4);
+  int64_t s = i / 36;
+  const double *p = aos + s * 72 + 18 * mu + 2 * k;
+  double2 v = make_double2(p[0], p[1]);
+  g18[((int64_t)mu * 9 + k) * V + s] = v;
+  if (k < 6) {
+    g12[((int64_t)mu * 6 + k) * V + s] = v;
+    g12f[((int64_t)mu * 6 + k) * V + s] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+}  // namespace cvchip
+
+using namespace cvchip;
+
+// =================================================================== C ABI
+
+extern "C" const char *cvchip_last_error(void) { return g_err; }
+
+extern "C" int cvchip_ctx_create(cvchip_ctx **ctx, const int dims[4], const double bc_angle[4],
+                                 double kappa, double mu) {
+  if (!ctx || !dims || !bc_angle) {
+    set_error("NULL argument to cvchip_ctx_create");
+    return CVCHIP_ERR_NULL;
+  }
+  for (int m = 0; m < 4; m++)
+    if (dims[m] < 2) {
+      set_error("lattice extent %d in direction %d is below 2", dims[m], m);
+      return CVCHIP_ERR_ARG;
+    }
+  int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  if (V * 256 > (int64_t)INT32_MAX * 4) {
+    set_error("lattice volume %lld too large", (long long)V);
+    return CVCHIP_ERR_ARG;
+  }
+  if (!(kappa > 0.0)) {
+    set_error("kappa must be positive");
+    return CVCHIP_ERR_ARG;
+  }
+  cvchip_ctx *c = new cvchip_ctx();
+  c->lat.T = dims[0];
+  c->lat.LX = dims[1];
+  c->lat.LY = dims[2];
+  c->lat.LZ = dims[3];
+  c->lat.V = V;
+  c->kappa = kappa;
+  c->mu = mu;
+  for (int m = 0; m < 4; m++) {
+    c->bc[m] = bc_angle[m];
+    // cvc_geometry.c:452-459: co_phase_up[mu] = exp(i pi BCangle[mu] / L_mu)
+    c->ph.re[m] = cos(bc_angle[m] * M_PI / (double)dims[m]);
+    c->ph.im[m] = sin(bc_angle[m] * M_PI / (double)dims[m]);
+  }
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    int rc = hip_fail(e, "context setup", __FILE__, __LINE__);
+    cvchip_ctx_destroy(c);
+    return rc;
+  }
+  *ctx = c;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_ctx_destroy(cvchip_ctx *c) {
+  if (!c) return CVCHIP_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gauge64) (void)hipFree(c->gauge64);
+  if (c->gauge64r12) (void)hipFree(c->gauge64r12);
+  if (c->gauge32r12) (void)hipFree(c->gauge32r12);
+  if (c->stage) (void)hipFree(c->stage);
+  if (c->red.ptr) (void)hipFree(c->red.ptr);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_ctx_set_params(cvchip_ctx *c, double kappa, double mu) {
+  if (!c) return CVCHIP_ERR_NULL;
+  if (!(kappa > 0.0)) {
+    set_error("kappa must be positive");
+    return CVCHIP_ERR_ARG;
+  }
+  c->kappa = kappa;
+  c->mu = mu;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_ctx_set_link_recon(cvchip_ctx *c, int recon) {
+  if (!c) return CVCHIP_ERR_NULL;
+  if (recon != 18 && recon != 12) {
+    set_error("link reconstruction must be 18 or 12, got %d", recon);
+    return CVCHIP_ERR_ARG;
+  }
+  c->link_recon = recon;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_set_gauge(cvchip_ctx *c, const double *gauge) {
+  if (!c || !gauge) {
+    set_error("NULL argument to cvchip_set_gauge");
+    return CVCHIP_ERR_NULL;
+  }
+  const int64_t V = c->lat.V;
+  if (!c->gauge64) CVC_HIP(hipMalloc(&c->gauge64, sizeof(double2) * 36 * V));
+  if (!c->gauge64r12) CVC_HIP(hipMalloc(&c->gauge64r12, sizeof(double2) * 24 * V));
+  if (!c->gauge32r12) CVC_HIP(hipMalloc(&c->gauge32r12, sizeof(float2) * 24 * V));
+  CVC_TRY(ensure_stage(c, sizeof(double) * 72 * V));
+  CVC_HIP(hipMemcpyAsync(c->stage, gauge, sizeof(double) * 72 * V, hipMemcpyHostToDevice, c->stream));
+  const int64_t n = 36 * V;
+  hipLaunchKernelGGL(k_gauge_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->gauge64, c->gauge64r12, c->gauge32r12, (const double *)c->stage, V);
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  c->gauge_set = 1;
+  return CVCHIP_OK;
+}
+
+namespace cvchip {
+
+int upload_aos(cvchip_ctx *c, Field &f, const double *host, int rhs0, int n) {
+  if (rhs0 < 0 || n < 0 || rhs0 + n > f.nrhs) {
+    set_error("rhs range %d..%d outside field of %d", rhs0, rhs0 + n, f.nrhs);
+    return CVCHIP_ERR_ARG;
+  }
+  if (n == 0) return 0;
+  const int64_t V = f.V;
+  const size_t bytes = sizeof(double) * 24 * V * n;
+  CVC_TRY(ensure_stage(c, bytes));
+  CVC_HIP(hipMemcpyAsync(c->stage, host, bytes, hipMemcpyHostToDevice, c->stream));
+  const int64_t tot = (int64_t)n * V * 12;
+  dim3 g((unsigned)((tot + 255) / 256)), b(256);
+  if (f.prec == CVCHIP_PREC_F64)
+    hipLaunchKernelGGL(k_aos_to_field<double>, g, b, 0, c->stream, (double2 *)f.data,
+                       (const double *)c->stage, V, f.nrhs, rhs0, n);
+  else if (f.prec == CVCHIP_PREC_F32)
+    hipLaunchKernelGGL(k_aos_to_field<float>, g, b, 0, c->stream, (float2 *)f.data,
+                       (const double *)c->stage, V, f.nrhs, rhs0, n);
+  else {
+    set_error("host upload into FP16 fields is not supported; upload FP32 and convert");
+    return CVCHIP_ERR_ARG;
+  }
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int download_aos(cvchip_ctx *c, const Field &f, double *host, int rhs0, int n) {
+  if (rhs0 < 0 || n < 0 || rhs0 + n > f.nrhs) {
+    set_error("rhs range %d..%d outside field of %d", rhs0, rhs0 + n, f.nrhs);
+    return CVCHIP_ERR_ARG;
+  }
+  if (n == 0) return 0;
+  const int64_t V = f.V;
+  const size_t bytes = sizeof(double) * 24 * V * n;
+  CVC_TRY(ensure_stage(c, bytes));
+  const int64_t tot = (int64_t)n * V * 12;
+  dim3 g((unsigned)((tot + 255) / 256)), b(256);
+  if (f.prec == CVCHIP_PREC_F64)
+    hipLaunchKernelGGL(k_field_to_aos<double>, g, b, 0, c->stream, (double *)c->stage,
+                       (const double2 *)f.data, V, f.nrhs, rhs0, n);
+  else if (f.prec == CVCHIP_PREC_F32)
+    hipLaunchKernelGGL(k_field_to_aos<float>, g, b, 0, c->stream, (double *)c->stage,
+                       (const float2 *)f.data, V, f.nrhs, rhs0, n);
+  else {
+    set_error("host download from FP16 fields is not supported");
+    return CVCHIP_ERR_ARG;
+  }
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipMemcpyAsync(host, c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace cvchip
+
+extern "C" int cvchip_field_create(cvchip_ctx *c, int nrhs, int prec, cvchip_field **out) {
+  if (!c || !out) return CVCHIP_ERR_NULL;
+  cvchip_field *f = new cvchip_field();
+  int rc = field_alloc(c, f->f, nrhs, prec);
+  if (rc) {
+    field_free(f->f);
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_field_destroy(cvchip_field *f) {
+  if (!f) return CVCHIP_OK;
+  field_free(f->f);
+  delete f;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_field_upload(cvchip_ctx *c, cvchip_field *f, const double *host, int rhs0, int n) {
+  if (!c || !f || !host) return CVCHIP_ERR_NULL;
+  return upload_aos(c, f->f, host, rhs0, n);
+}
+
+extern "C" int cvchip_field_download(cvchip_ctx *c, const cvchip_field *f, double *host, int rhs0, int n) {
+  if (!c || !f || !host) return CVCHIP_ERR_NULL;
+  return download_aos(c, f->f, host, rhs0, n);
+}

@@ -1,0 +1,160 @@
+// Internal definitions of libcvchip: context, device field layout, complex
+// and SU(3) helpers, the Dirac-matrix structure used by the kernels.
+//
+// Device layout (all precisions):
+//   spinor field set of N right-hand sides:  cplx[12][V][N]
+//     element (component c = 3*spin + colour, site s, rhs r) at (c*V + s)*N + r,
+//     so that the flat thread index i = s*N + r walks memory contiguously.
+//   gauge (recon-18):                        cplx[4][9][V]  (link mu, entry 3*i+j, site)
+//   gauge (recon-12):                        cplx[4][6][V]  (first two rows; the third
+//                                                             row is rebuilt in registers)
+// Sites are lexicographic with z fastest: s = ((t*LX + x)*LY + y)*LZ + z, as in
+// the reference (cvc_geometry.c:17-26).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+
+#include "../../include/cvchip.h"
+
+namespace cvchip {
+
+// ------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define CVC_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess) return ::cvchip::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define CVC_TRY(call)                                                        \
+  do {                                                                       \
+    int _rc = (call);                                                        \
+    if (_rc != 0) return _rc;                                                \
+  } while (0)
+
+// ------------------------------------------------------------- complex
+template <typename R> struct cx;
+template <> struct cx<double> { using t = double2; };
+template <> struct cx<float> { using t = float2; };
+template <typename R> using cplx = typename cx<R>::t;
+
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mk(R a, R b) {
+  cplx<R> c; c.x = a; c.y = b; return c;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { a.x -= b.x; a.y -= b.y; return a; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C c; c.x = a.x * b.x - a.y * b.y; c.y = a.x * b.y + a.y * b.x; return c;
+}
+// conj(a) * b
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
+  C c; c.x = a.x * b.x + a.y * b.y; c.y = a.x * b.y - a.y * b.x; return c;
+}
+template <typename C, typename R> __device__ __forceinline__ C cscale(C a, R s) { a.x *= s; a.y *= s; return a; }
+// a + b * c
+template <typename C> __device__ __forceinline__ C cfma(C a, C b, C c) {
+  a.x += b.x * c.x - b.y * c.y; a.y += b.x * c.y + b.y * c.x; return a;
+}
+// a + conj(b) * c
+template <typename C> __device__ __forceinline__ C cfmac(C a, C b, C c) {
+  a.x += b.x * c.x + b.y * c.y; a.y += b.x * c.y - b.y * c.x; return a;
+}
+// multiplication by a unit u in {+1, -1, +i, -i} (code 0,1,2,3), resolved at compile time
+template <int U, typename C> __device__ __forceinline__ C cunit(C a) {
+  if constexpr (U == 0) return a;
+  else if constexpr (U == 1) { a.x = -a.x; a.y = -a.y; return a; }
+  else if constexpr (U == 2) { C c; c.x = -a.y; c.y = a.x; return c; }
+  else { C c; c.x = a.y; c.y = -a.x; return c; }
+}
+
+// ------------------------------------------------------- Dirac matrices
+// gamma_mu of the reference basis (cvc_utils.c:709-745): one unit entry per
+// row.  GCOL[mu][a] is the column of the non-zero in row a, GVAL[mu][a] its
+// value as a unit code (0:+1 1:-1 2:+i 3:-i).  gamma5 = diag(1,1,-1,-1).
+constexpr int GCOL[4][4] = {{2, 3, 0, 1}, {3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}};
+constexpr int GVAL[4][4] = {{1, 1, 1, 1}, {3, 3, 2, 2}, {1, 0, 0, 1}, {3, 2, 2, 3}};
+constexpr int uneg(int u) { return u ^ 1; }   // -u
+
+// --------------------------------------------------------------- context
+struct Lattice {
+  int T, LX, LY, LZ;
+  int64_t V;
+};
+
+struct Phases { double re[4], im[4]; };
+
+struct Field {
+  int nrhs = 0;
+  int prec = CVCHIP_PREC_F64;
+  int64_t V = 0;
+  void *data = nullptr;      // cplx<R>[12][V][N]
+  float *norm = nullptr;     // FP16 storage: per (site, rhs) scale
+  size_t bytes = 0;
+};
+
+struct Scratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace cvchip
+
+struct cvchip_field {
+  cvchip::Field f;
+};
+
+struct cvchip_ctx {
+  cvchip::Lattice lat;
+  double kappa, mu;
+  double bc[4];
+  cvchip::Phases ph;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // gauge copies
+  double2 *gauge64 = nullptr;    // recon-18, FP64
+  double2 *gauge64r12 = nullptr; // recon-12, FP64
+  float2 *gauge32r12 = nullptr;  // recon-12, FP32
+  int gauge_set = 0;
+  int link_recon = 18;           // 18 or 12 for the FP64 Dslash
+  // staging for host <-> device transfers (AoS reference layout)
+  double *stage = nullptr;
+  size_t stage_bytes = 0;
+  // reduction scratch
+  cvchip::Scratch red;
+  // per-context solver workspaces are allocated per call
+};
+
+namespace cvchip {
+
+int ensure_stage(cvchip_ctx *c, size_t bytes);
+int ensure_red(cvchip_ctx *c, size_t bytes);
+int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec);
+void field_free(Field &f);
+
+// host AoS [V][24] (nrhs contiguous fields) <-> device field rhs range
+int upload_aos(cvchip_ctx *c, Field &f, const double *host, int rhs0, int n);
+int download_aos(cvchip_ctx *c, const Field &f, double *host, int rhs0, int n);
+
+// Dslash launcher: out = D(mutm) in (gamma5_out: out = gamma5 D in).
+// Optional fused reductions, written as per-block partials into c->red:
+//   DOT_NONE, DOT_TS (<out,in> complex and <out,out>), DOT_R2 (<r2,out> complex)
+enum DotMode { DOT_NONE = 0, DOT_TS = 1, DOT_R2 = 2 };
+int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_out,
+           DotMode dm = DOT_NONE, const Field *r2 = nullptr, int *nblocks_out = nullptr);
+
+// block-size convention shared by all per-rhs reductions
+inline int block_for(int nrhs) {
+  int s = 256 / nrhs;
+  if (s < 1) s = 1;
+  return s * nrhs;
+}
+
+}  // namespace cvchip

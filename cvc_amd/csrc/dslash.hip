@@ -1,0 +1,336 @@
+// Twisted-mass Wilson operator on gfx950:
+//   xi = D(mutm) phi = 1/(2 kappa) phi + i mutm gamma5 phi
+//        - 1/2 sum_mu [ (1 - gamma_mu) U~_mu(x) phi(x+mu) + (1 + gamma_mu) U~_mu(x-mu)^dagger phi(x-mu) ]
+// with U~_mu = U_mu * exp(i pi bc_mu / L_mu) (reference Q_phi.c:25-36, 38-188;
+// phases cvc_geometry.c:452-459).
+//
+// One thread per (site, rhs); the rhs index is the fastest-moving index of the
+// field layout, so the 64 lanes of a wave read 64 consecutive complex numbers
+// of each colour-spin plane and lanes of the same site share link loads.
+// Each hop projects the neighbour spinor to two spin components with
+// (1 -+ gamma_mu) (rank 2 in the chiral basis), multiplies the half spinor by
+// the link, and rebuilds the lower spin components from the upper ones.
+// Periodic wrap and the boundary phase are resolved in the kernel: there is no
+// halo.  Optional epilogues fuse the solver's per-rhs dot products.
+#include "spinor_io.h"
+
+namespace cvchip {
+
+template <typename R>
+struct DParams {
+  uint32_t V, VN, sX, sT;
+  int N, T, LX, LY, LZ;
+  R k2, mutm;
+  cplx<R> ph[4];
+};
+
+template <typename R, int RECON>
+__device__ __forceinline__ void load_link(cplx<R> u[9], const cplx<R> *__restrict__ U, int mu,
+                                          uint32_t s, uint32_t V) {
+  if constexpr (RECON == 18) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = U[(size_t)(mu * 9 + k) * V + s];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) u[k] = U[(size_t)(mu * 6 + k) * V + s];
+    // SU(3): row2 = conj(row0 x row1)
+    cplx<R> a = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
+    cplx<R> b = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
+    cplx<R> d = csub(cmul(u[0], u[4]), cmul(u[1], u[3]));
+    u[6] = mk<R>(a.x, -a.y);
+    u[7] = mk<R>(b.x, -b.y);
+    u[8] = mk<R>(d.x, -d.y);
+  }
+}
+
+// acc += (1 - gamma_MU) U ph psi   (FWD)   or   (1 + gamma_MU) U^dagger conj(ph) psi   (!FWD)
+template <typename R, int MU, bool FWD>
+__device__ __forceinline__ void hop(cplx<R> acc[12], const cplx<R> psi[12], const cplx<R> u[9],
+                                    cplx<R> ph) {
+  using C = cplx<R>;
+  constexpr int p0 = GCOL[MU][0], p1 = GCOL[MU][1];
+  constexpr int g0 = FWD ? uneg(GVAL[MU][0]) : GVAL[MU][0];
+  constexpr int g1 = FWD ? uneg(GVAL[MU][1]) : GVAL[MU][1];
+  C h[2][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    h[0][c] = cadd(psi[c], cunit<g0>(psi[3 * p0 + c]));
+    h[1][c] = cadd(psi[3 + c], cunit<g1>(psi[3 * p1 + c]));
+  }
+  C v[2][3];
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      C s = mk<R>(0, 0);
+      if constexpr (FWD) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) s = cfma(s, u[3 * i + j], h[a][j]);
+        v[a][i] = cmul(ph, s);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 3; j++) s = cfmac(s, u[3 * j + i], h[a][j]);
+        v[a][i] = cmulc(ph, s);
+      }
+    }
+  }
+  // lower components: (1 -+ gamma) psi at row b equals -+g_b h[p(b)]
+  constexpr int q2 = GCOL[MU][2], q3 = GCOL[MU][3];
+  constexpr int l2 = FWD ? uneg(GVAL[MU][2]) : GVAL[MU][2];
+  constexpr int l3 = FWD ? uneg(GVAL[MU][3]) : GVAL[MU][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    acc[i] = cadd(acc[i], v[0][i]);
+    acc[3 + i] = cadd(acc[3 + i], v[1][i]);
+    acc[6 + i] = cadd(acc[6 + i], cunit<l2>(v[q2][i]));
+    acc[9 + i] = cadd(acc[9 + i], cunit<l3>(v[q3][i]));
+  }
+}
+
+template <typename IO, int RECON, int MU>
+__device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const IO &in,
+                                         const cplx<typename IO::real> *__restrict__ U,
+                                         const DParams<typename IO::real> &P, uint32_t s, uint32_t r,
+                                         int coord, int L, uint32_t stride) {
+  using R = typename IO::real;
+  cplx<R> psi[12], u[9];
+  const uint32_t sm = coord == 0 ? s + (uint32_t)(L - 1) * stride : s - stride;
+  const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
+  // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
+  in.load(psi, sm * P.N + r);
+  load_link<R, RECON>(u, U, MU, sm, P.V);
+  hop<R, MU, false>(acc, psi, u, P.ph[MU]);
+  // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
+  in.load(psi, sp * P.N + r);
+  load_link<R, RECON>(u, U, MU, s, P.V);
+  hop<R, MU, true>(acc, psi, u, P.ph[MU]);
+}
+
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM>
+__global__ void __launch_bounds__(256)
+k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParams<typename IOI::real> P,
+         IOI r2, double *__restrict__ part) {
+  using R = typename IOI::real;
+  using C = cplx<R>;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dv[3] = {0., 0., 0.};
+  if (i < P.VN) {
+    const uint32_t s = i / (uint32_t)P.N;
+    const uint32_t r = i - s * (uint32_t)P.N;
+    const uint32_t t = s / P.sT;
+    uint32_t rem = s - t * P.sT;
+    const uint32_t x = rem / P.sX;
+    rem -= x * P.sX;
+    const uint32_t y = rem / (uint32_t)P.LZ;
+    const uint32_t z = rem - y * (uint32_t)P.LZ;
+
+    C acc[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) acc[c] = mk<R>(0, 0);
+    hop_pair<IOI, RECON, 0>(acc, in, U, P, s, r, (int)t, P.T, P.sT);
+    hop_pair<IOI, RECON, 1>(acc, in, U, P, s, r, (int)x, P.LX, P.sX);
+    hop_pair<IOI, RECON, 2>(acc, in, U, P, s, r, (int)y, P.LY, (uint32_t)P.LZ);
+    hop_pair<IOI, RECON, 3>(acc, in, U, P, s, r, (int)z, P.LZ, 1u);
+
+    C own[12];
+    in.load(own, i);
+    const R mh = (R)-0.5;
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      // -1/2 hop + phi/(2 kappa) + i mutm gamma5 phi  (gamma5 = +1 on spins 0,1, -1 on 2,3)
+      const R m = c < 6 ? P.mutm : -P.mutm;
+      C o;
+      o.x = mh * acc[c].x + P.k2 * own[c].x - m * own[c].y;
+      o.y = mh * acc[c].y + P.k2 * own[c].y + m * own[c].x;
+      if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
+      acc[c] = o;
+    }
+    out.store(acc, i);
+    if constexpr (DM == DOT_TS) {
+      // <out, in> = sum conj(out) in ; <out, out>
+      R a = 0, b = 0, n = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        a += acc[c].x * own[c].x + acc[c].y * own[c].y;
+        b += acc[c].x * own[c].y - acc[c].y * own[c].x;
+        n += acc[c].x * acc[c].x + acc[c].y * acc[c].y;
+      }
+      dv[0] = a; dv[1] = b; dv[2] = n;
+    } else if constexpr (DM == DOT_R2) {
+      C w[12];
+      r2.load(w, i);
+      R a = 0, b = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        a += w[c].x * acc[c].x + w[c].y * acc[c].y;
+        b += w[c].x * acc[c].y - w[c].y * acc[c].x;
+      }
+      dv[0] = a; dv[1] = b;
+    }
+  }
+  if constexpr (DM == DOT_TS) block_reduce_store<3>(dv, part, P.N);
+  else if constexpr (DM == DOT_R2) block_reduce_store<2>(dv, part, P.N);
+}
+
+// ------------------------------------------------------------- launcher
+
+template <typename R>
+static DParams<R> make_params(const cvchip_ctx *c, int N, double mutm) {
+  DParams<R> P;
+  P.V = (uint32_t)c->lat.V;
+  P.N = N;
+  P.VN = (uint32_t)(c->lat.V * N);
+  P.T = c->lat.T;
+  P.LX = c->lat.LX;
+  P.LY = c->lat.LY;
+  P.LZ = c->lat.LZ;
+  P.sX = (uint32_t)(c->lat.LY * c->lat.LZ);
+  P.sT = (uint32_t)(c->lat.LX * c->lat.LY * c->lat.LZ);
+  P.k2 = (R)(0.5 / c->kappa);
+  P.mutm = (R)mutm;
+  for (int m = 0; m < 4; m++) P.ph[m] = mk<R>((R)c->ph.re[m], (R)c->ph.im[m]);
+  return P;
+}
+
+template <typename IO> static IO make_io(const Field &f);
+template <> IOPlain<double> make_io<IOPlain<double>>(const Field &f) {
+  return IOPlain<double>{(double2 *)f.data, (uint32_t)(f.V * f.nrhs)};
+}
+template <> IOPlain<float> make_io<IOPlain<float>>(const Field &f) {
+  return IOPlain<float>{(float2 *)f.data, (uint32_t)(f.V * f.nrhs)};
+}
+template <> IOHalf make_io<IOHalf>(const Field &f) {
+  return IOHalf{(__half2 *)f.data, f.norm, (uint32_t)(f.V * f.nrhs)};
+}
+
+template <typename IOI, typename IOO, int RECON>
+static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g5, DotMode dm,
+                  const Field *r2, int *nblk_out) {
+  using R = typename IOI::real;
+  const int N = in.nrhs;
+  DParams<R> P = make_params<R>(c, N, mutm);
+  const int B = block_for(N);
+  const unsigned nblk = (unsigned)((P.VN + B - 1) / B);
+  double *part = nullptr;
+  if (dm != DOT_NONE) {
+    CVC_TRY(ensure_red(c, sizeof(double) * ((size_t)nblk * 3 * N + 64 * 256)));
+    part = (double *)c->red.ptr;
+  }
+  const cplx<R> *U;
+  if constexpr (sizeof(R) == 8) U = (RECON == 18) ? c->gauge64 : c->gauge64r12;
+  else U = c->gauge32r12;
+  IOO o = make_io<IOO>(out);
+  IOI ii = make_io<IOI>(in);
+  IOI rr = r2 ? make_io<IOI>(*r2) : ii;
+  dim3 g(nblk), b(B);
+#define CVC_LAUNCH(G5, DMV) \
+  hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV>), g, b, 0, c->stream, o, ii, U, P, rr, part)
+  if (g5) {
+    if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);
+    else if (dm == DOT_TS) CVC_LAUNCH(true, DOT_TS);
+    else CVC_LAUNCH(true, DOT_R2);
+  } else {
+    if (dm == DOT_NONE) CVC_LAUNCH(false, DOT_NONE);
+    else if (dm == DOT_TS) CVC_LAUNCH(false, DOT_TS);
+    else CVC_LAUNCH(false, DOT_R2);
+  }
+#undef CVC_LAUNCH
+  CVC_HIP(hipGetLastError());
+  if (nblk_out) *nblk_out = (int)nblk;
+  return 0;
+}
+
+int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_out, DotMode dm,
+           const Field *r2, int *nblocks_out) {
+  if (!c->gauge_set) {
+    set_error("gauge field not set (call cvchip_set_gauge first)");
+    return CVCHIP_ERR_STATE;
+  }
+  if (out.nrhs != in.nrhs || out.V != in.V || (r2 && (r2->nrhs != in.nrhs || r2->prec != in.prec))) {
+    set_error("operator fields do not match (nrhs %d vs %d)", out.nrhs, in.nrhs);
+    return CVCHIP_ERR_ARG;
+  }
+  if (out.data == in.data) {
+    set_error("operator output must not alias its input");
+    return CVCHIP_ERR_ARG;
+  }
+  if (in.prec == CVCHIP_PREC_F64 && out.prec == CVCHIP_PREC_F64) {
+    if (c->link_recon == 12)
+      return launch<IOPlain<double>, IOPlain<double>, 12>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+    return launch<IOPlain<double>, IOPlain<double>, 18>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+  }
+  if (in.prec == CVCHIP_PREC_F32 && out.prec == CVCHIP_PREC_F32)
+    return launch<IOPlain<float>, IOPlain<float>, 12>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+  if (in.prec == CVCHIP_PREC_F16 && out.prec == CVCHIP_PREC_F16)
+    return launch<IOHalf, IOHalf, 12>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+  set_error("unsupported operator precision pair %d -> %d", in.prec, out.prec);
+  return CVCHIP_ERR_ARG;
+}
+
+// ------------------------------------------------------- reductions
+
+__global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int Q, int N,
+                                  double *__restrict__ res) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x / N, r = blockIdx.x % N;
+  double a = 0.;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[((size_t)b * Q + q) * N + r];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[q * N + r] = sh[0];
+}
+
+int reduce_partials(cvchip_ctx *c, const double *part, int nblk, int Q, int N, double *res) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(Q * N), dim3(256), 0, c->stream, part, nblk, Q, N, res);
+  CVC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace cvchip
+
+using namespace cvchip;
+
+extern "C" int cvchip_field_dslash(cvchip_ctx *c, cvchip_field *out, const cvchip_field *in,
+                                   double mutm, int gamma5_out) {
+  if (!c || !out || !in) return CVCHIP_ERR_NULL;
+  CVC_TRY(dslash(c, out->f, in->f, mutm, gamma5_out));
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_field_dslash_timed(cvchip_ctx *c, cvchip_field *out, const cvchip_field *in,
+                                         double mutm, int reps, double *ms) {
+  if (!c || !out || !in || !ms) return CVCHIP_ERR_NULL;
+  if (reps < 1) {
+    set_error("reps must be positive");
+    return CVCHIP_ERR_ARG;
+  }
+  CVC_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int k = 0; k < reps; k++) CVC_TRY(dslash(c, out->f, in->f, mutm, 0));
+  CVC_HIP(hipEventRecord(c->ev1, c->stream));
+  CVC_HIP(hipEventSynchronize(c->ev1));
+  float t = 0.f;
+  CVC_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = (double)t / reps;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_Q_phi(cvchip_ctx *c, double *xi, const double *phi, double mutm, int gamma5_out) {
+  if (!c || !xi || !phi) {
+    set_error("NULL argument to cvchip_Q_phi");
+    return CVCHIP_ERR_NULL;
+  }
+  Field a, b;
+  int rc = field_alloc(c, a, 1, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, b, 1, CVCHIP_PREC_F64);
+  if (!rc) rc = upload_aos(c, a, phi, 0, 1);
+  if (!rc) rc = dslash(c, b, a, mutm, gamma5_out);
+  if (!rc) rc = download_aos(c, b, xi, 0, 1);
+  field_free(a);
+  field_free(b);
+  return rc;
+}

@@ -1,0 +1,962 @@
+// BiCGStab for D(mu) x = b on N right-hand sides at once (reference
+// invert_Qtm.c:86-270, invert_Qtm_her.c:276-478), device resident.
+//
+// Per iteration (r0rn, alpha, omega, beta per rhs):
+//   alpha = r0rn / <r2,p2>                      (scalar kernel)
+//   s     = r1 - alpha p2                       (vector kernel)
+//   t     = D s,   <t,s>, <t,t>                 (operator with fused dot epilogue)
+//   omega = <t,s> / <t,t>                       (scalar kernel)
+//   r1    = s - omega t,  <r1,r1>, <r2,r1>      (vector kernel with dots)
+//   test  |r1|^2 <= sp |b|^2; beta = (<r2,r1>/r0rn)(alpha/omega); r0rn = <r2,r1>
+//   x    += omega s + alpha p ;  p = r1 + beta (p - omega p2)   (one pass)
+//   p2    = D p,  <r2,p2>                       (operator with fused dot epilogue)
+// Dot products are per-thread sums of a site's 12 components, block trees per
+// rhs and a fixed-order tree over blocks: results are reproducible run to run.
+// A rhs that has converged is frozen (its vectors are no longer written).
+// The reference returns the iterate from before the last update
+// (invert_Qtm.c:199-211); CVCHIP_SOLVE_LAGGED reproduces that, the default
+// applies the update so that x matches the tested residual.
+//
+// The mixed-precision solver runs the same recurrence on FP32 (or FP16 +
+// norm) vectors and, for each rhs, replaces the iterated residual by the true
+// FP64 residual b - D x whenever |r| has dropped by the factor delta since the
+// previous such update (reliable updates), accumulating the low-precision
+// solution into the FP64 x at each update.  Convergence is always decided on
+// the FP64 true residual.
+#include <stdio.h>
+#include <chrono>
+#include <vector>
+
+#include "spinor_io.h"
+
+namespace cvchip {
+
+struct RhsState {
+  double alpha[2], omega[2], beta[2], r0rn[2], r0rn_prev[2];
+  double normb, norm, tol2;
+  double rmax;        // mixed: |r|^2 at the last reliable update
+  double true_res2;
+  int done, final_upd, iter, upd;
+};
+
+// ------------------------------------------------------------ vector kernels
+
+template <typename IO> static IO mkio(const Field &f);
+template <> IOPlain<double> mkio<IOPlain<double>>(const Field &f) {
+  return IOPlain<double>{(double2 *)f.data, (uint32_t)(f.V * f.nrhs)};
+}
+template <> IOPlain<float> mkio<IOPlain<float>>(const Field &f) {
+  return IOPlain<float>{(float2 *)f.data, (uint32_t)(f.V * f.nrhs)};
+}
+template <> IOHalf mkio<IOHalf>(const Field &f) {
+  return IOHalf{(__half2 *)f.data, f.norm, (uint32_t)(f.V * f.nrhs)};
+}
+
+#define CVC_ELEMWISE_PROLOGUE                                  \
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;    \
+  const int rhs = (int)(threadIdx.x % (unsigned)N);            \
+  const bool act = i < VN;                                     \
+  (void)rhs;
+
+// s = r1 - alpha p2
+template <typename IO>
+__global__ void __launch_bounds__(256) k_s_update(IO s, IO r1, IO p2, const RhsState *__restrict__ st,
+                                                  uint32_t VN, int N) {
+  CVC_ELEMWISE_PROLOGUE
+  if (!act || st[rhs].done) return;
+  using R = typename IO::real;
+  using C = cplx<R>;
+  const C a = mk<R>((R)st[rhs].alpha[0], (R)st[rhs].alpha[1]);
+  C vr[12], vp[12];
+  r1.load(vr, i);
+  p2.load(vp, i);
+#pragma unroll
+  for (int c = 0; c < 12; c++) vr[c] = csub(vr[c], cmul(a, vp[c]));
+  s.store(vr, i);
+}
+
+// r1 = s - omega t ; partial dots <r1,r1>, <r2,r1>
+template <typename IO>
+__global__ void __launch_bounds__(256) k_r1_update(IO r1, IO s, IO t, IO r2, const RhsState *__restrict__ st,
+                                                   uint32_t VN, int N, double *__restrict__ part) {
+  CVC_ELEMWISE_PROLOGUE
+  using R = typename IO::real;
+  using C = cplx<R>;
+  double dv[3] = {0., 0., 0.};
+  if (act && !st[rhs].done) {
+    const C w = mk<R>((R)st[rhs].omega[0], (R)st[rhs].omega[1]);
+    C vs[12], vt[12], v2[12];
+    s.load(vs, i);
+    t.load(vt, i);
+    r2.load(v2, i);
+    R n = 0, a = 0, b = 0;
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      C o = csub(vs[c], cmul(w, vt[c]));
+      vs[c] = o;
+      n += o.x * o.x + o.y * o.y;
+      a += v2[c].x * o.x + v2[c].y * o.y;
+      b += v2[c].x * o.y - v2[c].y * o.x;
+    }
+    r1.store(vs, i);
+    dv[0] = n; dv[1] = a; dv[2] = b;
+  }
+  block_reduce_store<3>(dv, part, N);
+}
+
+// x += omega s + alpha p  (while running, or once more on convergence when
+// not lagged); p = r1 + beta (p - omega p2) (while running)
+template <typename IOX, typename IO>
+__global__ void __launch_bounds__(256) k_xp_update(IOX x, IO p, IO s, IO r1, IO p2,
+                                                   const RhsState *__restrict__ st, uint32_t VN, int N) {
+  CVC_ELEMWISE_PROLOGUE
+  if (!act) return;
+  const RhsState &S = st[rhs];
+  const bool upx = !S.done;               // includes the final (non-lagged) update
+  const bool upp = !S.done && !S.final_upd && !S.upd;
+  if (!upx) return;
+  using R = typename IO::real;
+  using C = cplx<R>;
+  using RX = typename IOX::real;
+  using CX = cplx<RX>;
+  const C a = mk<R>((R)S.alpha[0], (R)S.alpha[1]);
+  const C w = mk<R>((R)S.omega[0], (R)S.omega[1]);
+  C vp[12], vs[12];
+  CX vx[12];
+  p.load(vp, i);
+  s.load(vs, i);
+  x.load(vx, i);
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    C d = cadd(cmul(w, vs[c]), cmul(a, vp[c]));
+    vx[c].x += (RX)d.x;
+    vx[c].y += (RX)d.y;
+  }
+  x.store(vx, i);
+  if (upp) {
+    const C be = mk<R>((R)S.beta[0], (R)S.beta[1]);
+    C vr[12], v2[12];
+    r1.load(vr, i);
+    p2.load(v2, i);
+#pragma unroll
+    for (int c = 0; c < 12; c++) vp[c] = cadd(vr[c], cmul(be, csub(vp[c], cmul(w, v2[c]))));
+    p.store(vp, i);
+  }
+}
+
+// r = b - t  (t = D x), optionally copied to r2 and p; partial dots <r,r>, <b,b>
+template <typename IOB, typename IO>
+__global__ void __launch_bounds__(256) k_residual(IO r, IO r2, IO p, IOB b, IOB t, int copies,
+                                                  const RhsState *__restrict__ st, int only_upd,
+                                                  uint32_t VN, int N, double *__restrict__ part) {
+  CVC_ELEMWISE_PROLOGUE
+  using RB = typename IOB::real;
+  using CB = cplx<RB>;
+  using R = typename IO::real;
+  using C = cplx<R>;
+  double dv[2] = {0., 0.};
+  if (act && (!only_upd || st[rhs].upd)) {
+    CB vb[12], vt[12];
+    b.load(vb, i);
+    t.load(vt, i);
+    C o[12];
+    double n = 0., nb = 0.;
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      CB d = csub(vb[c], vt[c]);
+      n += (double)d.x * d.x + (double)d.y * d.y;
+      nb += (double)vb[c].x * vb[c].x + (double)vb[c].y * vb[c].y;
+      o[c] = mk<R>((R)d.x, (R)d.y);
+    }
+    r.store(o, i);
+    if (copies) {
+      r2.store(o, i);
+      p.store(o, i);
+    }
+    dv[0] = n; dv[1] = nb;
+  }
+  block_reduce_store<2>(dv, part, N);
+}
+
+// dst = src (precision conversion), optionally only rhs flagged in st[].upd
+template <typename IOD, typename IOS>
+__global__ void __launch_bounds__(256) k_copy(IOD d, IOS s, const RhsState *__restrict__ st, int only_upd,
+                                              uint32_t VN, int N) {
+  CVC_ELEMWISE_PROLOGUE
+  if (!act || (only_upd && !st[rhs].upd)) return;
+  using RS = typename IOS::real;
+  using RD = typename IOD::real;
+  cplx<RS> v[12];
+  cplx<RD> o[12];
+  s.load(v, i);
+#pragma unroll
+  for (int c = 0; c < 12; c++) o[c] = mk<RD>((RD)v[c].x, (RD)v[c].y);
+  d.store(o, i);
+}
+
+// x64 += xl (low precision accumulator), xl = 0, for rhs flagged in st[].upd
+template <typename IOX, typename IOL>
+__global__ void __launch_bounds__(256) k_accum_zero(IOX x, IOL xl, const RhsState *__restrict__ st,
+                                                    uint32_t VN, int N) {
+  CVC_ELEMWISE_PROLOGUE
+  if (!act || !st[rhs].upd) return;
+  cplx<double> vx[12];
+  cplx<typename IOL::real> vl[12];
+  x.load(vx, i);
+  xl.load(vl, i);
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    vx[c].x += (double)vl[c].x;
+    vx[c].y += (double)vl[c].y;
+    vl[c] = mk<typename IOL::real>(0, 0);
+  }
+  x.store(vx, i);
+  xl.store(vl, i);
+}
+
+// <a,b> complex and <a,a>: partials for Q=3
+template <typename IO>
+__global__ void __launch_bounds__(256) k_dot3(IO a, IO b, uint32_t VN, int N, double *__restrict__ part) {
+  CVC_ELEMWISE_PROLOGUE
+  double dv[3] = {0., 0., 0.};
+  if (act) {
+    cplx<typename IO::real> va[12], vb[12];
+    a.load(va, i);
+    b.load(vb, i);
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      dv[0] += (double)va[c].x * vb[c].x + (double)va[c].y * vb[c].y;
+      dv[1] += (double)va[c].x * vb[c].y - (double)va[c].y * vb[c].x;
+      dv[2] += (double)va[c].x * va[c].x + (double)va[c].y * va[c].y;
+    }
+  }
+  block_reduce_store<3>(dv, part, N);
+}
+
+// ------------------------------------------------------------ scalar kernels
+
+// res = reduced dots of the preceding kernel (layout [q][N]).
+__global__ void k_sc_init(RhsState *st, const double *res, int N, double sp, int mixed) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  S.norm = res[r];
+  S.normb = res[N + r];
+  S.tol2 = sp * S.normb;
+  S.r0rn[0] = S.norm;
+  S.r0rn[1] = 0.;
+  S.rmax = S.norm;
+  S.final_upd = 0;
+  S.upd = 0;
+  S.iter = -1;
+  S.done = (S.norm <= S.tol2) ? 1 : 0;       // invert_Qtm.c:142-147: return 0
+  if (S.done) S.iter = 0;
+  S.alpha[0] = S.alpha[1] = S.omega[0] = S.omega[1] = S.beta[0] = S.beta[1] = 0.;
+  (void)mixed;
+}
+
+__device__ __forceinline__ void cdiv(double o[2], const double a[2], const double b[2]) {
+  const double d = b[0] * b[0] + b[1] * b[1];
+  const double re = (a[0] * b[0] + a[1] * b[1]) / d;
+  const double im = (a[1] * b[0] - a[0] * b[1]) / d;
+  o[0] = re;
+  o[1] = im;
+}
+
+// alpha = r0rn / <r2,p2>
+__global__ void k_sc_alpha(RhsState *st, const double *res, int N) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (S.final_upd) {
+    S.done = 1;
+    S.final_upd = 0;
+  }
+  if (S.done) return;
+  const double w[2] = {res[r], res[N + r]};
+  cdiv(S.alpha, S.r0rn, w);
+}
+
+// omega = <t,s>/<t,t>
+__global__ void k_sc_omega(RhsState *st, const double *res, int N) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (S.done) return;
+  const double inv = 1. / res[2 * N + r];
+  S.omega[0] = res[r] * inv;
+  S.omega[1] = res[N + r] * inv;
+}
+
+// after r1: convergence test (or reliable-update trigger), beta, r0rn
+__global__ void k_sc_beta(RhsState *st, const double *res, int N, int it, int lagged, int mixed,
+                          double delta2) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (S.done) return;
+  S.norm = res[r];
+  const double w[2] = {res[N + r], res[2 * N + r]};
+  if (!mixed) {
+    if (S.norm <= S.tol2) {
+      S.iter = it;
+      if (lagged) S.done = 1;
+      else S.final_upd = 1;
+      if (S.done) return;
+    }
+  } else {
+    // reliable update when the iterated residual fell by delta since the last
+    // update, or reached the tolerance (then decided in FP64)
+    S.upd = (S.norm <= delta2 * S.rmax || S.norm <= S.tol2) ? 1 : 0;
+  }
+  double w2[2], w3[2];
+  cdiv(w2, w, S.r0rn);
+  cdiv(w3, S.alpha, S.omega);
+  S.beta[0] = w2[0] * w3[0] - w2[1] * w3[1];
+  S.beta[1] = w2[0] * w3[1] + w2[1] * w3[0];
+  S.r0rn_prev[0] = S.r0rn[0];
+  S.r0rn_prev[1] = S.r0rn[1];
+  S.r0rn[0] = w[0];
+  S.r0rn[1] = w[1];
+}
+
+// mixed: after the FP64 residual recompute (res: <r,r>, unused) decide convergence
+__global__ void k_sc_reliable(RhsState *st, const double *res, int N, int it) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (!S.upd) return;
+  S.norm = res[r];
+  S.rmax = S.norm;
+  if (S.norm <= S.tol2) {
+    S.done = 1;
+    S.iter = it;
+  }
+}
+
+// mixed: after a reliable update, beta = (<r2,r>/r0rn_prev)(alpha/omega) and
+// r0rn = <r2,r> with the replaced residual r (res: <r2,r> complex)
+__global__ void k_sc_r0rn(RhsState *st, const double *res, int N) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (!S.upd || S.done) return;
+  const double w[2] = {res[r], res[N + r]};
+  double w2[2], w3[2];
+  cdiv(w2, w, S.r0rn_prev);
+  cdiv(w3, S.alpha, S.omega);
+  S.beta[0] = w2[0] * w3[0] - w2[1] * w3[1];
+  S.beta[1] = w2[0] * w3[1] + w2[1] * w3[0];
+  S.r0rn[0] = w[0];
+  S.r0rn[1] = w[1];
+}
+
+__global__ void k_sc_clear_upd(RhsState *st, int N) {
+  const int r = threadIdx.x;
+  if (r < N) st[r].upd = 0;
+}
+
+// mixed, rhs flagged for a reliable update: p = r + beta (p - omega p2)
+template <typename IO>
+__global__ void __launch_bounds__(256) k_p_update(IO p, IO r1, IO p2, const RhsState *__restrict__ st,
+                                                  uint32_t VN, int N) {
+  CVC_ELEMWISE_PROLOGUE
+  if (!act || !st[rhs].upd || st[rhs].done) return;
+  using R = typename IO::real;
+  using C = cplx<R>;
+  const C w = mk<R>((R)st[rhs].omega[0], (R)st[rhs].omega[1]);
+  const C be = mk<R>((R)st[rhs].beta[0], (R)st[rhs].beta[1]);
+  C vp[12], vr[12], v2[12];
+  p.load(vp, i);
+  r1.load(vr, i);
+  p2.load(v2, i);
+#pragma unroll
+  for (int c = 0; c < 12; c++) vp[c] = cadd(vr[c], cmul(be, csub(vp[c], cmul(w, v2[c]))));
+  p.store(vp, i);
+}
+
+// ------------------------------------------------------------------ driver
+
+struct Solve {
+  cvchip_ctx *c;
+  int N;
+  uint32_t VN;
+  int B;
+  unsigned nblk;
+  RhsState *st = nullptr;     // device
+  double *res = nullptr;      // device [4][N]
+  RhsState *hst = nullptr;    // pinned host mirror
+};
+
+static int solve_setup(cvchip_ctx *c, Solve &S, int N) {
+  S.c = c;
+  S.N = N;
+  S.VN = (uint32_t)(c->lat.V * N);
+  S.B = block_for(N);
+  S.nblk = (S.VN + S.B - 1) / S.B;
+  CVC_TRY(ensure_red(c, sizeof(double) * ((size_t)S.nblk * 3 * N + 64 * 256)));
+  CVC_HIP(hipMalloc(&S.st, sizeof(RhsState) * N));
+  CVC_HIP(hipMalloc(&S.res, sizeof(double) * 4 * N));
+  CVC_HIP(hipHostMalloc(&S.hst, sizeof(RhsState) * N, hipHostMallocDefault));
+  CVC_HIP(hipMemsetAsync(S.st, 0, sizeof(RhsState) * N, c->stream));
+  return 0;
+}
+
+static void solve_free(Solve &S) {
+  if (S.st) (void)hipFree(S.st);
+  if (S.res) (void)hipFree(S.res);
+  if (S.hst) (void)hipHostFree(S.hst);
+}
+
+static int reduce(Solve &S, int nblk, int Q) {
+  return reduce_partials(S.c, (const double *)S.c->red.ptr, nblk, Q, S.N, S.res);
+}
+
+static int pull_state(Solve &S) {
+  CVC_HIP(hipMemcpyAsync(S.hst, S.st, sizeof(RhsState) * S.N, hipMemcpyDeviceToHost, S.c->stream));
+  CVC_HIP(hipStreamSynchronize(S.c->stream));
+  return 0;
+}
+
+#define LAUNCH_EW(kern, ...)                                                                     \
+  do {                                                                                           \
+    hipLaunchKernelGGL(kern, dim3(S.nblk), dim3(S.B), 0, c->stream, __VA_ARGS__);                \
+    CVC_HIP(hipGetLastError());                                                                  \
+  } while (0)
+
+// true residual |D x - b|^2 / |b|^2 per rhs into stats / state
+static int true_residual(Solve &S, Field &x, const Field &b, Field &tmp, double mutm, double *maxres,
+                         std::vector<double> *per_rhs) {
+  cvchip_ctx *c = S.c;
+  CVC_TRY(dslash(c, tmp, x, mutm, 0));
+  IOPlain<double> ib = mkio<IOPlain<double>>(b), it = mkio<IOPlain<double>>(tmp);
+  // tmp <- b - D x (sign irrelevant for the norm)
+  LAUNCH_EW((k_residual<IOPlain<double>, IOPlain<double>>), it, it, it, ib, it, 0, S.st, 0, S.VN, S.N,
+            (double *)c->red.ptr);
+  CVC_TRY(reduce(S, (int)S.nblk, 2));
+  std::vector<double> h(2 * S.N);
+  CVC_HIP(hipMemcpyAsync(h.data(), S.res, sizeof(double) * 2 * S.N, hipMemcpyDeviceToHost, c->stream));
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  double m = 0.;
+  if (per_rhs) per_rhs->resize(S.N);
+  for (int r = 0; r < S.N; r++) {
+    double v = h[S.N + r] > 0 ? h[r] / h[S.N + r] : 0.;
+    if (v > m) m = v;
+    if (per_rhs) (*per_rhs)[r] = v;
+  }
+  *maxres = m;
+  return 0;
+}
+
+// The operator of the recurrence: D(mu), or for the gamma5-hermitian system
+// A = gamma5 D(mu) gamma5 D(-mu) = Qf5(mu) Qf5(-mu) (invert_Qtm.c:325-327).
+struct Op {
+  cvchip_ctx *c;
+  int her;
+  Field *aux;
+  int apply(Field &out, const Field &in, DotMode dm, const Field *r2, int *nb) {
+    if (!her) return dslash(c, out, in, c->mu, 0, dm, r2, nb);
+    CVC_TRY(dslash(c, *aux, in, -c->mu, 1));
+    return dslash(c, out, *aux, c->mu, 1, dm, r2, nb);
+  }
+};
+
+// gamma5 on all rhs of a FP64 set (in place)
+__global__ void k_g5(double2 *__restrict__ f, uint32_t VN) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= VN) return;
+#pragma unroll
+  for (int c = 6; c < 12; c++) {
+    double2 v = f[(size_t)c * VN + i];
+    v.x = -v.x;
+    v.y = -v.y;
+    f[(size_t)c * VN + i] = v;
+  }
+}
+
+// FP64 BiCGStab (on the gamma5-hermitian system when her != 0; then b is
+// expected as gamma5 b and x receives y of A y = gamma5 b)
+static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, unsigned flags,
+                      cvchip_stats *stats, int her = 0) {
+  const int N = b.nrhs;
+  Solve S;
+  int rc = solve_setup(c, S, N);
+  Field r1, r2, s, t, p, p2, aux;
+  Field *all[7] = {&r1, &r2, &s, &t, &p, &p2, &aux};
+  for (int k = 0; k < 6 + (her ? 1 : 0) && !rc; k++) rc = field_alloc(c, *all[k], N, CVCHIP_PREC_F64);
+  auto cleanup = [&]() {
+    for (int k = 0; k < 7; k++) field_free(*all[k]);
+    solve_free(S);
+  };
+  Op op{c, her, &aux};
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  using IO = IOPlain<double>;
+  IO ir1 = mkio<IO>(r1), ir2 = mkio<IO>(r2), is = mkio<IO>(s), iT = mkio<IO>(t), ip = mkio<IO>(p),
+     ip2 = mkio<IO>(p2), ix = mkio<IO>(x), ib = mkio<IO>(b);
+  const int lagged = (flags & CVCHIP_SOLVE_LAGGED) ? 1 : 0;
+  const int verbose = (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0;
+  int dcalls = 0, nb = 0;
+  int iters = 0, all_done = 0;
+
+#define TRY(x)                                                                                   \
+  do {                                                                                           \
+    rc = (x);                                                                                    \
+    if (rc) {                                                                                    \
+      cleanup();                                                                                 \
+      return rc;                                                                                 \
+    }                                                                                            \
+  } while (0)
+
+  // r = b - D x ; r2 = p = r ; <r,r>, <b,b>
+  TRY(op.apply(t, x, DOT_NONE, nullptr, nullptr));
+  dcalls++;
+  hipLaunchKernelGGL((k_residual<IO, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib, iT, 1,
+                     S.st, 0, S.VN, N, (double *)c->red.ptr);
+  TRY(reduce(S, (int)S.nblk, 2));
+  hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, sp, 0);
+  TRY(pull_state(S));
+  all_done = 1;
+  for (int r = 0; r < N; r++) all_done &= S.hst[r].done;
+  if (!all_done) {
+    TRY(op.apply(p2, p, DOT_R2, &r2, &nb));
+    dcalls++;
+    TRY(reduce(S, nb, 2));
+    for (int it = 0; it <= niter_max; it++) {
+      hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+      hipLaunchKernelGGL((k_s_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, is, ir1, ip2, S.st, S.VN, N);
+      TRY(op.apply(t, s, DOT_TS, nullptr, &nb));
+      dcalls++;
+      TRY(reduce(S, nb, 3));
+      hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+      hipLaunchKernelGGL((k_r1_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, is, iT, ir2, S.st,
+                         S.VN, N, (double *)c->red.ptr);
+      TRY(reduce(S, (int)S.nblk, 3));
+      hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it, lagged, 0, 0.0);
+      hipLaunchKernelGGL((k_xp_update<IO, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ix, ip, is, ir1, ip2,
+                         S.st, S.VN, N);
+      CVC_HIP(hipGetLastError());
+      iters = it + 1;
+      TRY(pull_state(S));
+      all_done = 1;
+      for (int r = 0; r < N; r++) all_done &= (S.hst[r].done | S.hst[r].final_upd);
+      if (verbose)
+        for (int r = 0; r < N; r++)
+          printf("# [%d] residuum after iteration %d: %25.16e\n", r, it, S.hst[r].norm);
+      if (all_done) break;
+      TRY(op.apply(p2, p, DOT_R2, &r2, &nb));
+      dcalls++;
+      TRY(reduce(S, nb, 2));
+    }
+  }
+  // final state (final_upd counts as converged)
+  TRY(pull_state(S));
+  int nconv = 0, maxit = 0;
+  for (int r = 0; r < N; r++) {
+    if (S.hst[r].done || S.hst[r].final_upd) {
+      nconv++;
+      if (S.hst[r].iter > maxit) maxit = S.hst[r].iter;
+    }
+  }
+  double tr = 0.;
+  if (!her) {
+    TRY(true_residual(S, x, b, t, c->mu, &tr, nullptr));
+    dcalls++;
+  }
+  if (stats) {
+    stats->iterations = iters;
+    stats->converged = nconv;
+    stats->reliable_updates = 0;
+    stats->dslash_calls = dcalls;
+    stats->true_res2_max = tr;
+  }
+  cleanup();
+#undef TRY
+  if (nconv < N) {
+    set_error("BiCGStab: %d of %d right-hand sides not converged after %d iterations", N - nconv, N,
+              niter_max + 1);
+    return CVCHIP_ERR_NOCONV;
+  }
+  return maxit;
+}
+
+// Mixed precision: FP32 or FP16 recurrence, FP64 reliable updates (on the
+// gamma5-hermitian system when her != 0, b then being gamma5 b).
+template <typename IOL>
+static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec,
+                          double delta, cvchip_stats *stats, int her = 0) {
+  const int N = b.nrhs;
+  Solve S;
+  int rc = solve_setup(c, S, N);
+  Field r1, r2, s, t, p, p2, xl, t64, auxl, aux64;
+  Field *lo[7] = {&r1, &r2, &s, &t, &p, &p2, &xl};
+  for (int k = 0; k < 6 && !rc; k++) rc = field_alloc(c, *lo[k], N, prec);
+  // the solution accumulator since the last reliable update stays FP32
+  if (!rc) rc = field_alloc(c, xl, N, CVCHIP_PREC_F32);
+  if (!rc) rc = field_alloc(c, t64, N, CVCHIP_PREC_F64);
+  if (!rc && her) rc = field_alloc(c, auxl, N, prec);
+  if (!rc && her) rc = field_alloc(c, aux64, N, CVCHIP_PREC_F64);
+  auto cleanup = [&]() {
+    for (int k = 0; k < 7; k++) field_free(*lo[k]);
+    field_free(t64);
+    field_free(auxl);
+    field_free(aux64);
+    solve_free(S);
+  };
+  Op opl{c, her, &auxl}, op64{c, her, &aux64};
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  using IO = IOL;
+  using I64 = IOPlain<double>;
+  IO ir1 = mkio<IO>(r1), ir2 = mkio<IO>(r2), is = mkio<IO>(s), iT = mkio<IO>(t), ip = mkio<IO>(p),
+     ip2 = mkio<IO>(p2);
+  IOPlain<float> ixl = mkio<IOPlain<float>>(xl);
+  I64 ix = mkio<I64>(x), ib = mkio<I64>(b), it64 = mkio<I64>(t64);
+  const double delta2 = delta * delta;
+  int dcalls = 0, nb = 0, nrel = 0, iters = 0, all_done = 0;
+
+#define TRY(x)                                                                                   \
+  do {                                                                                           \
+    rc = (x);                                                                                    \
+    if (rc) {                                                                                    \
+      cleanup();                                                                                 \
+      return rc;                                                                                 \
+    }                                                                                            \
+  } while (0)
+
+  // r = b - D x in FP64, converted; r2 = p = r; xl = 0
+  TRY(op64.apply(t64, x, DOT_NONE, nullptr, nullptr));
+  dcalls++;
+  hipLaunchKernelGGL((k_residual<I64, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib, it64, 1,
+                     S.st, 0, S.VN, N, (double *)c->red.ptr);
+  TRY(reduce(S, (int)S.nblk, 2));
+  hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, sp, 1);
+  CVC_HIP(hipMemsetAsync(xl.data, 0, xl.bytes, c->stream));
+  TRY(pull_state(S));
+  all_done = 1;
+  for (int r = 0; r < N; r++) all_done &= S.hst[r].done;
+  if (!all_done) {
+    TRY(opl.apply(p2, p, DOT_R2, &r2, &nb));
+    dcalls++;
+    TRY(reduce(S, nb, 2));
+    for (int it = 0; it <= niter_max; it++) {
+      hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+      hipLaunchKernelGGL((k_s_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, is, ir1, ip2, S.st, S.VN, N);
+      TRY(opl.apply(t, s, DOT_TS, nullptr, &nb));
+      dcalls++;
+      TRY(reduce(S, nb, 3));
+      hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+      hipLaunchKernelGGL((k_r1_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, is, iT, ir2, S.st,
+                         S.VN, N, (double *)c->red.ptr);
+      TRY(reduce(S, (int)S.nblk, 3));
+      hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it, 0, 1, delta2);
+      // xl += omega s + alpha p ; p = r1 + beta (p - omega p2)
+      hipLaunchKernelGGL((k_xp_update<IOPlain<float>, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ixl, ip, is, ir1, ip2,
+                         S.st, S.VN, N);
+      CVC_HIP(hipGetLastError());
+      iters = it + 1;
+      TRY(pull_state(S));
+      int any_upd = 0;
+      for (int r = 0; r < N; r++) any_upd |= (!S.hst[r].done && S.hst[r].upd);
+      if (any_upd) {
+        // reliable update for flagged rhs: x += xl; xl = 0; r = b - D64 x
+        nrel++;
+        hipLaunchKernelGGL((k_accum_zero<I64, IOPlain<float>>), dim3(S.nblk), dim3(S.B), 0, c->stream, ix, ixl, S.st,
+                           S.VN, N);
+        TRY(op64.apply(t64, x, DOT_NONE, nullptr, nullptr));
+        dcalls++;
+        hipLaunchKernelGGL((k_residual<I64, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib,
+                           it64, 0, S.st, 1, S.VN, N, (double *)c->red.ptr);
+        TRY(reduce(S, (int)S.nblk, 2));
+        hipLaunchKernelGGL(k_sc_reliable, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it);
+        // r0rn and beta with the replaced residual: <r2, r>
+        hipLaunchKernelGGL((k_dot3<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir2, ir1, S.VN, N,
+                           (double *)c->red.ptr);
+        TRY(reduce(S, (int)S.nblk, 3));
+        hipLaunchKernelGGL(k_sc_r0rn, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+        hipLaunchKernelGGL((k_p_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ip, ir1, ip2, S.st, S.VN, N);
+        hipLaunchKernelGGL(k_sc_clear_upd, dim3(1), dim3(256), 0, c->stream, S.st, N);
+        CVC_HIP(hipGetLastError());
+        TRY(pull_state(S));
+      }
+      all_done = 1;
+      for (int r = 0; r < N; r++) all_done &= S.hst[r].done;
+      if (all_done) break;
+      TRY(opl.apply(p2, p, DOT_R2, &r2, &nb));
+      dcalls++;
+      TRY(reduce(S, nb, 2));
+    }
+  }
+  TRY(pull_state(S));
+  int nconv = 0, maxit = 0;
+  for (int r = 0; r < N; r++)
+    if (S.hst[r].done) {
+      nconv++;
+      if (S.hst[r].iter > maxit) maxit = S.hst[r].iter;
+    }
+  // fold in what the low-precision accumulator still holds (non-converged rhs)
+  double tr = 0.;
+  if (!her) {
+    TRY(true_residual(S, x, b, t64, c->mu, &tr, nullptr));
+    dcalls++;
+  }
+  if (stats) {
+    stats->iterations = iters;
+    stats->converged = nconv;
+    stats->reliable_updates = nrel;
+    stats->dslash_calls = dcalls;
+    stats->true_res2_max = tr;
+  }
+  cleanup();
+#undef TRY
+  if (nconv < N) {
+    set_error("mixed BiCGStab: %d of %d right-hand sides not converged after %d iterations", N - nconv, N,
+              niter_max + 1);
+    return CVCHIP_ERR_NOCONV;
+  }
+  return maxit;
+}
+
+// invert_Qtm_her (invert_Qtm.c:276-478): y from A y = gamma5 b, then x = gamma5 D(-mu) y.
+int invert_her_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec,
+                      double delta, unsigned flags, cvchip_stats *stats) {
+  if (!c->gauge_set) {
+    set_error("gauge field not set (call cvchip_set_gauge first)");
+    return CVCHIP_ERR_STATE;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  Field g5b, y;
+  int rc = field_alloc(c, g5b, b.nrhs, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, y, b.nrhs, CVCHIP_PREC_F64);
+  if (!rc) {
+    const uint32_t VN = (uint32_t)(b.V * b.nrhs);
+    rc = hipMemcpyAsync(g5b.data, b.data, b.bytes, hipMemcpyDeviceToDevice, c->stream) == hipSuccess ? 0 : CVCHIP_ERR_HIP;
+    if (!rc && hipMemcpyAsync(y.data, x.data, x.bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) rc = CVCHIP_ERR_HIP;
+    if (!rc) {
+      hipLaunchKernelGGL(k_g5, dim3((VN + 255) / 256), dim3(256), 0, c->stream, (double2 *)g5b.data, VN);
+      if (prec == CVCHIP_PREC_F64) rc = bicgstab64(c, y, g5b, sp, niter_max, flags | CVCHIP_SOLVE_LAGGED, stats, 1);
+      else if (!(delta > 0. && delta < 1.)) {
+        set_error("reliable-update factor delta must lie in (0,1), got %g", delta);
+        rc = CVCHIP_ERR_ARG;
+      } else if (prec == CVCHIP_PREC_F32)
+        rc = bicgstab_mixed<IOPlain<float>>(c, y, g5b, sp, niter_max, prec, delta, stats, 1);
+      else if (prec == CVCHIP_PREC_F16)
+        rc = bicgstab_mixed<IOHalf>(c, y, g5b, sp, niter_max, prec, delta, stats, 1);
+      else {
+        set_error("unknown precision %d", prec);
+        rc = CVCHIP_ERR_ARG;
+      }
+    }
+    // x = gamma5 D(-mu) y, done whether or not the solve converged (invert_Qtm.c:441-446)
+    if (rc >= 0 || rc == CVCHIP_ERR_NOCONV) {
+      const int rc2 = dslash(c, x, y, -c->mu, 1);
+      if (rc2) rc = rc2;
+    }
+    if (stats && (rc >= 0 || rc == CVCHIP_ERR_NOCONV)) {
+      Solve S;
+      double tr = 0.;
+      Field tmp;
+      int rc3 = solve_setup(c, S, b.nrhs);
+      if (!rc3) rc3 = field_alloc(c, tmp, b.nrhs, CVCHIP_PREC_F64);
+      if (!rc3) rc3 = true_residual(S, x, b, tmp, c->mu, &tr, nullptr);
+      field_free(tmp);
+      solve_free(S);
+      if (rc3) rc = rc3;
+      stats->true_res2_max = tr;
+      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  }
+  field_free(g5b);
+  field_free(y);
+  return rc;
+}
+
+int invert_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec, double delta,
+                  unsigned flags, cvchip_stats *stats) {
+  if (!c->gauge_set) {
+    set_error("gauge field not set (call cvchip_set_gauge first)");
+    return CVCHIP_ERR_STATE;
+  }
+  if (x.prec != CVCHIP_PREC_F64 || b.prec != CVCHIP_PREC_F64 || x.nrhs != b.nrhs) {
+    set_error("solution and source must be FP64 sets of equal size");
+    return CVCHIP_ERR_ARG;
+  }
+  if (niter_max < 0 || !(sp >= 0.)) {
+    set_error("invalid niter_max %d or tolerance %g", niter_max, sp);
+    return CVCHIP_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int rc;
+  if (prec == CVCHIP_PREC_F64) rc = bicgstab64(c, x, b, sp, niter_max, flags, stats);
+  else {
+    if (!(delta > 0. && delta < 1.)) {
+      set_error("reliable-update factor delta must lie in (0,1), got %g", delta);
+      return CVCHIP_ERR_ARG;
+    }
+    if (prec == CVCHIP_PREC_F32) rc = bicgstab_mixed<IOPlain<float>>(c, x, b, sp, niter_max, prec, delta, stats);
+    else if (prec == CVCHIP_PREC_F16) rc = bicgstab_mixed<IOHalf>(c, x, b, sp, niter_max, prec, delta, stats);
+    else {
+      set_error("unknown precision %d", prec);
+      return CVCHIP_ERR_ARG;
+    }
+  }
+  if (stats)
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+}  // namespace cvchip
+
+using namespace cvchip;
+
+extern "C" int cvchip_field_invert(cvchip_ctx *c, cvchip_field *x, const cvchip_field *b, double sp,
+                                   int niter_max, int prec, double delta, unsigned flags,
+                                   cvchip_stats *stats) {
+  if (!c || !x || !b) {
+    set_error("NULL argument to cvchip_field_invert");
+    return CVCHIP_ERR_NULL;
+  }
+  return invert_fields(c, x->f, b->f, sp, niter_max, prec, delta, flags, stats);
+}
+
+extern "C" int cvchip_invert_Qtm(cvchip_ctx *c, double *x, const double *b, int nrhs, double sp,
+                                 int niter_max, int prec, double delta, unsigned flags, cvchip_stats *stats) {
+  if (!c || !x || !b) {
+    set_error("NULL argument to cvchip_invert_Qtm");
+    return CVCHIP_ERR_NULL;
+  }
+  Field fx, fb;
+  int rc = field_alloc(c, fx, nrhs, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, fb, nrhs, CVCHIP_PREC_F64);
+  if (!rc) rc = upload_aos(c, fx, x, 0, nrhs);
+  if (!rc) rc = upload_aos(c, fb, b, 0, nrhs);
+  int ret = rc;
+  if (!rc) {
+    ret = invert_fields(c, fx, fb, sp, niter_max, prec, delta, flags, stats);
+    // the reference leaves x as computed also when not converged
+    if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
+      int rc2 = download_aos(c, fx, x, 0, nrhs);
+      if (rc2) ret = rc2;
+    }
+  }
+  field_free(fx);
+  field_free(fb);
+  return ret;
+}
+
+extern "C" int cvchip_invert_Qtm_her(cvchip_ctx *c, double *x, const double *b, int nrhs, double sp,
+                                     int niter_max, int prec, double delta, unsigned flags,
+                                     cvchip_stats *stats) {
+  if (!c || !x || !b) {
+    set_error("NULL argument to cvchip_invert_Qtm_her");
+    return CVCHIP_ERR_NULL;
+  }
+  Field fx, fb;
+  int rc = field_alloc(c, fx, nrhs, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, fb, nrhs, CVCHIP_PREC_F64);
+  if (!rc) rc = upload_aos(c, fx, x, 0, nrhs);
+  if (!rc) rc = upload_aos(c, fb, b, 0, nrhs);
+  int ret = rc;
+  if (!rc) {
+    ret = invert_her_fields(c, fx, fb, sp, niter_max, prec, delta, flags, stats);
+    if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
+      int rc2 = download_aos(c, fx, x, 0, nrhs);
+      if (rc2) ret = rc2;
+    }
+  }
+  field_free(fx);
+  field_free(fb);
+  return ret;
+}
+
+// spinor_scalar_product_co / _re (invert_Qtm.c:28-84) on host arrays
+static int scalar_product(cvchip_ctx *c, double w[3], const double *xi, const double *phi) {
+  Field a, b;
+  Solve S;
+  int rc = field_alloc(c, a, 1, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, b, 1, CVCHIP_PREC_F64);
+  if (!rc) rc = upload_aos(c, a, xi, 0, 1);
+  if (!rc) rc = upload_aos(c, b, phi, 0, 1);
+  if (!rc) rc = solve_setup(c, S, 1);
+  if (!rc) {
+    using IO = IOPlain<double>;
+    hipLaunchKernelGGL((k_dot3<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, mkio<IO>(a), mkio<IO>(b), S.VN, 1,
+                       (double *)c->red.ptr);
+    rc = reduce(S, (int)S.nblk, 3);
+    if (!rc) {
+      hipError_t e = hipMemcpyAsync(w, S.res, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "scalar product", __FILE__, __LINE__);
+    }
+  }
+  solve_free(S);
+  field_free(a);
+  field_free(b);
+  return rc;
+}
+
+extern "C" int cvchip_scalar_product_co(cvchip_ctx *c, double w[2], const double *xi, const double *phi) {
+  if (!c || !w || !xi || !phi) return CVCHIP_ERR_NULL;
+  double r[3];
+  CVC_TRY(scalar_product(c, r, xi, phi));
+  w[0] = r[0];
+  w[1] = r[1];
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_scalar_product_re(cvchip_ctx *c, double *out, const double *xi, const double *phi) {
+  if (!c || !out || !xi || !phi) return CVCHIP_ERR_NULL;
+  double r[3];
+  CVC_TRY(scalar_product(c, r, xi, phi));
+  *out = r[0];
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_field_invert_her(cvchip_ctx *c, cvchip_field *x, const cvchip_field *b, double sp,
+                                       int niter_max, int prec, double delta, unsigned flags,
+                                       cvchip_stats *stats) {
+  if (!c || !x || !b) {
+    set_error("NULL argument to cvchip_field_invert_her");
+    return CVCHIP_ERR_NULL;
+  }
+  if (x->f.prec != CVCHIP_PREC_F64 || b->f.prec != CVCHIP_PREC_F64 || x->f.nrhs != b->f.nrhs) {
+    set_error("solution and source must be FP64 sets of equal size");
+    return CVCHIP_ERR_ARG;
+  }
+  return invert_her_fields(c, x->f, b->f, sp, niter_max, prec, delta, flags, stats);
+}
+
+extern "C" int cvchip_field_copy(cvchip_ctx *c, cvchip_field *dst, const cvchip_field *src) {
+  if (!c || !dst || !src) return CVCHIP_ERR_NULL;
+  Field &d = dst->f;
+  const Field &s = src->f;
+  if (d.nrhs != s.nrhs || d.V != s.V) {
+    set_error("field copy between sets of different size");
+    return CVCHIP_ERR_ARG;
+  }
+  const int N = s.nrhs;
+  const uint32_t VN = (uint32_t)(s.V * N);
+  const int B = block_for(N);
+  dim3 g((VN + B - 1) / B), b(B);
+  using I64 = IOPlain<double>;
+  using I32 = IOPlain<float>;
+#define CP(TD, TS) hipLaunchKernelGGL((k_copy<TD, TS>), g, b, 0, c->stream, mkio<TD>(d), mkio<TS>(s), \
+                                      (const RhsState *)nullptr, 0, VN, N)
+  if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F64) CP(I64, I64);
+  else if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F32) CP(I64, I32);
+  else if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F16) CP(I64, IOHalf);
+  else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F64) CP(I32, I64);
+  else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F32) CP(I32, I32);
+  else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F16) CP(I32, IOHalf);
+  else if (d.prec == CVCHIP_PREC_F16 && s.prec == CVCHIP_PREC_F64) CP(IOHalf, I64);
+  else if (d.prec == CVCHIP_PREC_F16 && s.prec == CVCHIP_PREC_F32) CP(IOHalf, I32);
+  else CP(IOHalf, IOHalf);
+#undef CP
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
+}

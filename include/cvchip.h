@@ -1,0 +1,181 @@
+/*
+ * cvchip.h -- C ABI of the MI355X (gfx950) implementation of CVC's
+ * propagator-and-contraction path.
+ *
+ * Each entry point replaces a reference function (file:line given beside it).
+ * Conventions shared by all entry points:
+ *   - host arrays use the reference's layouts (global.h:55-61):
+ *       spinor  double[V][24]      2*(3*spin+colour)+{re,im}, site ((t*LX+x)*LY+y)*LZ+z
+ *       gauge   double[V][4][18]   18*mu + 2*(3*i+j)+{re,im}, mu = t,x,y,z
+ *       conn    double[V][16][2]   2*(16*ix + 4*mu + nu)+{re,im}
+ *     nrhs fields are passed as nrhs consecutive V*24 arrays;
+ *   - the boundary phase exp(i*pi*bc_angle[mu]/L_mu) multiplies every link
+ *     (cvc_geometry.c:452-459), resolved inside the kernels; no halo exchange;
+ *   - return 0 on success; -2 for NULL buffers and -3 for a solve that did not
+ *     converge, as invert_Qtm (invert_Qtm.c:112-113, 250); values <= -100
+ *     are argument / HIP errors, described by cvchip_last_error();
+ *   - calls are synchronous at the boundary and not thread-safe per context.
+ */
+#ifndef CVCHIP_H
+#define CVCHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVCHIP_OK            0
+#define CVCHIP_ERR_NULL     (-2)
+#define CVCHIP_ERR_NOCONV   (-3)
+#define CVCHIP_ERR_ARG      (-100)
+#define CVCHIP_ERR_HIP      (-101)
+#define CVCHIP_ERR_OOM      (-102)
+#define CVCHIP_ERR_STATE    (-103)
+
+/* arithmetic / storage precision of a field or of a solve */
+#define CVCHIP_PREC_F64   0
+#define CVCHIP_PREC_F32   1
+#define CVCHIP_PREC_F16   2   /* half storage + per-site float norm, FP32 arithmetic */
+
+/* solver flags */
+#define CVCHIP_SOLVE_LAGGED   1u  /* return the iterate from before the last update,
+                                     as invert_Qtm.c:199-211 does */
+#define CVCHIP_SOLVE_VERBOSE  2u  /* print the reference's per-iteration residual lines */
+
+typedef struct cvchip_ctx cvchip_ctx;
+typedef struct cvchip_field cvchip_field;
+
+typedef struct {
+  int    iterations;        /* BiCGStab iterations (largest over the right-hand sides)     */
+  int    converged;         /* number of right-hand sides that met the tolerance           */
+  int    reliable_updates;  /* FP64 residual recomputations (mixed precision)              */
+  int    dslash_calls;      /* operator applications in all precisions                     */
+  double true_res2_max;     /* max over rhs of |D x - b|^2 / |b|^2, in FP64                 */
+  double seconds;           /* wall time inside the call                                   */
+} cvchip_stats;
+
+/* ------------------------------------------------------------ context */
+
+/* Lattice geometry and parameters: init_geometry (cvc_geometry.c:343-465) plus
+ * the g_kappa / g_mu / BCangle globals (global.h:91-206).  dims = T,LX,LY,LZ. */
+int cvchip_ctx_create(cvchip_ctx **ctx, const int dims[4], const double bc_angle[4],
+                      double kappa, double mu);
+int cvchip_ctx_destroy(cvchip_ctx *ctx);
+int cvchip_ctx_set_params(cvchip_ctx *ctx, double kappa, double mu);
+/* link storage of the FP64 operator: 18 (full) or 12 (two rows, third rebuilt) */
+int cvchip_ctx_set_link_recon(cvchip_ctx *ctx, int recon);
+const char *cvchip_last_error(void);
+
+/* Upload g_gauge_field (read_lime_gauge_field_doubleprec, io.c:202-347, fills it
+ * on the reference side).  Must be called before any operator. */
+int cvchip_set_gauge(cvchip_ctx *ctx, const double *gauge);
+
+/* ------------------------------------------------- host-array operators */
+
+/* xi = D(mutm) phi with D = 1/(2 kappa) + i mutm gamma5 - 1/2 sum_mu [...]:
+ * Q_phi_tbc (Q_phi.c:38-188) with mutm = mu, Q_phi (Q_phi.c:1092-1242) for any
+ * mutm; gamma5_out = 1 gives Qf5 (Q_phi.c:789-934): xi = gamma5 D(mutm) phi. */
+int cvchip_Q_phi(cvchip_ctx *ctx, double *xi, const double *phi, double mutm, int gamma5_out);
+
+/* spinor_scalar_product_co / _re (invert_Qtm.c:28-84): sum_x xi(x)^dagger phi(x). */
+int cvchip_scalar_product_co(cvchip_ctx *ctx, double w[2], const double *xi, const double *phi);
+int cvchip_scalar_product_re(cvchip_ctx *ctx, double *r, const double *xi, const double *phi);
+
+/* invert_Qtm (invert_Qtm.c:86-270): BiCGStab for D(mu) x = b on nrhs right-hand
+ * sides solved together.  x is in/out (initial guess in).  sp is the squared
+ * relative tolerance solver_precision (|r|^2 <= sp |b|^2).  prec selects the
+ * FP64 solver or the mixed solver (FP32 / FP16 inner iteration with FP64
+ * reliable updates whenever |r| falls below delta times its value at the last
+ * update).  Returns the iteration count (largest over rhs), -2 or -3. */
+int cvchip_invert_Qtm(cvchip_ctx *ctx, double *x, const double *b, int nrhs, double sp,
+                      int niter_max, int prec, double delta, unsigned flags,
+                      cvchip_stats *stats);
+
+/* invert_Qtm_her (invert_Qtm.c:276-478): BiCGStab on gamma5 D(mu) gamma5 D(-mu)
+ * y = gamma5 b (= D(-mu)^dagger D(-mu) y, Hermitian positive definite),
+ * returning x = gamma5 D(-mu) y, which solves D(mu) x = b.  x is the initial
+ * guess for y on entry, as in the reference.  prec / delta as for
+ * cvchip_invert_Qtm (the mixed solver runs on the same Hermitian system). */
+int cvchip_invert_Qtm_her(cvchip_ctx *ctx, double *x, const double *b, int nrhs, double sp,
+                          int niter_max, int prec, double delta, unsigned flags,
+                          cvchip_stats *stats);
+
+/* ---------------------------------------------- device-resident fields */
+
+/* A set of nrhs spinor fields kept in HBM in the library's layout. */
+int cvchip_field_create(cvchip_ctx *ctx, int nrhs, int prec, cvchip_field **f);
+int cvchip_field_destroy(cvchip_field *f);
+int cvchip_field_upload(cvchip_ctx *ctx, cvchip_field *f, const double *host, int rhs0, int n);
+int cvchip_field_download(cvchip_ctx *ctx, const cvchip_field *f, double *host, int rhs0, int n);
+/* dst = src, converting precision (FP64 / FP32 / FP16+norm); equal nrhs. */
+int cvchip_field_copy(cvchip_ctx *ctx, cvchip_field *dst, const cvchip_field *src);
+/* out = D(mutm) in (gamma5_out: gamma5 D in) on all rhs of the set; precision
+ * of the sets selects the kernel. */
+int cvchip_field_dslash(cvchip_ctx *ctx, cvchip_field *out, const cvchip_field *in,
+                        double mutm, int gamma5_out);
+/* Launches the same operator reps times back to back on the library's stream
+ * and returns the average duration per launch from HIP events on that stream. */
+int cvchip_field_dslash_timed(cvchip_ctx *ctx, cvchip_field *out, const cvchip_field *in,
+                              double mutm, int reps, double *ms_per_launch);
+/* Device-resident invert_Qtm: solves D(mu) x = b for all rhs of b into x. */
+int cvchip_field_invert(cvchip_ctx *ctx, cvchip_field *x, const cvchip_field *b, double sp,
+                        int niter_max, int prec, double delta, unsigned flags,
+                        cvchip_stats *stats);
+
+/* Device-resident invert_Qtm_her. */
+int cvchip_field_invert_her(cvchip_ctx *ctx, cvchip_field *x, const cvchip_field *b, double sp,
+                            int niter_max, int prec, double delta, unsigned flags,
+                            cvchip_stats *stats);
+
+/* ---------------------------------------------------------- observables */
+
+/* Conserved-vector-current contraction of cvc_exact2_xspace.c:577-861 for the
+ * source point src = (t,x,y,z).  props: 120 host spinor fields ordered as the
+ * reference's g_spinor_field (cvc_exact2_xspace.c:401-430): index
+ * 12*s + 3*spin + colour for the up flavour, 60 + ... for the down flavour,
+ * with s = 0..3 the source y + s^ and s = 4 the source y.  conn receives the
+ * normalised (-1/4, :861) field, contact the 4 complex contact terms
+ * (contact_term, :697-704, 837-844). */
+int cvchip_cvc_contract(cvchip_ctx *ctx, double *conn, double contact[8],
+                        const double *const *props, const int src[4]);
+/* The same from device-resident propagators: up and dn each hold 60 rhs in the
+ * reference's order (12*s + 3*spin + colour). */
+int cvchip_field_cvc_contract(cvchip_ctx *ctx, double *conn, double contact[8],
+                              const cvchip_field *up, const cvchip_field *dn,
+                              const int src[4]);
+/* Ward identity of cvc_exact2_xspace.c:987-1006: wi[V][4][2],
+ * wi_nu(x) = sum_mu conn_{mu nu}(x) - conn_{mu nu}(x - mu). */
+int cvchip_wi(cvchip_ctx *ctx, double *wi, const double *conn);
+
+/* Momentum-space polarisation tensor (no reference counterpart; the reference
+ * writes Pi(x) only, cvc_exact2_xspace.c:876-882):
+ *   Pi_{mu nu}(q) = sum_x exp(-i q.(x - y)) exp(-i (q_mu - q_nu)/2) conn_{mu nu}(x)
+ *                   + delta_{mu nu} * contact_mu * (-1/2)... see cvchip_pi_q notes in
+ *   the implementation; q_mu = 2 pi n_mu / L_mu with n in [0, L_mu).
+ * piq: double[V][16][2] in the site order of n. */
+int cvchip_pi_q(cvchip_ctx *ctx, double *piq, const double *conn, const double contact[8],
+                const int src[4]);
+
+/* The whole path for one source point: 60 up + 60 down solves (mu and -mu)
+ * from point sources at y and y + nu^, then the contraction.  Replaces the
+ * propagator reads and contraction of cvc_exact2_xspace.c:401-861 with
+ * propagators computed on the GPU. */
+int cvchip_cvc_source(cvchip_ctx *ctx, double *conn, double contact[8], const int src[4],
+                      double sp, int niter_max, int prec, double delta,
+                      cvchip_stats *stats);
+
+/* -------------------------------------------------------------- inputs */
+
+/* Seeded gauge field U_mu(x) = exp(i eps H) in SU(3), H traceless Hermitian
+ * from complex Gaussian entries drawn with Philox4x32-10 keyed by (seed, site,
+ * mu).  Host-only; deterministic across runs and hosts of this image.  No
+ * reference counterpart (the reference's generators use rand(),
+ * cvc_utils.c:2275-2680). */
+int cvchip_gauge_random(const int dims[4], uint64_t seed, double eps, double *gauge);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,136 @@
+"""Parity of the HIP operator, vector ops and solvers with the oracle (GPU).
+
+Tolerances: FP64 operator 1e-13 relative (different summation order and
+FMA); FP32 operator 2e-6; FP16+norm storage 2e-3; solves compare true
+residuals against the requested squared tolerance and solutions against the
+oracle's at 1e-8 relative (both solved to a squared residual of 1e-24).
+"""
+import numpy as np
+import pytest
+
+from conftest import rand_spinor, rel_err
+
+pytestmark = pytest.mark.gpu
+
+BC = (1.0, 0.0, 0.0, 0.0)
+
+
+@pytest.fixture(scope="module")
+def lat8(chip, orc):
+    dims = (8, 8, 8, 8)
+    g = chip.gauge_random(dims, 1, 0.3)
+    return dims, g
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (6, 4, 8, 4), (8, 8, 8, 8)])
+@pytest.mark.parametrize("recon", [18, 12])
+def test_dslash_f64(chip, orc, dims, recon):
+    print(f"dslash f64 dims={dims} recon={recon}", flush=True)
+    g = chip.gauge_random(dims, 2, 0.3)
+    bc = (1.0, 0.3, 0.0, -0.5)
+    ctx = chip.Context(dims, 0.163, 0.01, bc, gauge=g)
+    ctx.set_link_recon(recon)
+    L = orc.Lattice(dims, 0.163, 0.01, bc)
+    rng = np.random.default_rng(3)
+    phi = rand_spinor(rng, L.V)
+    assert rel_err(ctx.Q_phi(phi), L.Q_phi(g, phi)) < 1e-13                       # Q_phi_tbc
+    assert rel_err(ctx.Q_phi(phi, mutm=-0.2), L.Q_phi(g, phi, -0.2)) < 1e-13      # Q_phi
+    assert rel_err(ctx.Q_phi(phi, mutm=0.05, gamma5_out=True), L.Qf5(g, phi, 0.05)) < 1e-13  # Qf5
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 12])
+def test_dslash_multi_rhs(chip, orc, nrhs):
+    dims = (4, 6, 4, 8)
+    print(f"dslash multi-rhs nrhs={nrhs}", flush=True)
+    g = chip.gauge_random(dims, 5, 0.3)
+    ctx = chip.Context(dims, 0.1632, 0.004, BC, gauge=g)
+    L = orc.Lattice(dims, 0.1632, 0.004, BC)
+    rng = np.random.default_rng(nrhs)
+    phi = rand_spinor(rng, L.V, nrhs)
+    fin, fout = ctx.field(nrhs), ctx.field(nrhs)
+    fin.upload(phi)
+    ctx.dslash(fout, fin)
+    got = fout.download()
+    for r in range(nrhs):
+        assert rel_err(got[r], L.Q_phi(g, phi[r])) < 1e-13
+    # reduced precisions through conversion
+    for prec, tol in [(chip.PREC_F32, 2e-6), (chip.PREC_F16, 2e-3)]:
+        a, b, back = ctx.field(nrhs, prec), ctx.field(nrhs, prec), ctx.field(nrhs)
+        a.copy_from(fin)
+        ctx.dslash(b, a)
+        back.copy_from(b)
+        got = back.download()
+        for r in range(nrhs):
+            assert rel_err(got[r], L.Q_phi(g, phi[r])) < tol, (prec, r)
+
+
+def test_scalar_products(chip, orc):
+    dims = (4, 4, 4, 4)
+    g = chip.gauge_random(dims, 1, 0.3)
+    ctx = chip.Context(dims, 0.15, 0.01, BC, gauge=g)
+    L = orc.Lattice(dims, 0.15, 0.01, BC)
+    rng = np.random.default_rng(4)
+    a, b = rand_spinor(rng, L.V), rand_spinor(rng, L.V)
+    want = L.dot_co(a, b)
+    assert abs(ctx.scalar_product_co(a, b) - want) < 1e-12 * abs(want)
+    assert abs(ctx.scalar_product_re(a, b) - want.real) < 1e-12 * abs(want)
+
+
+def test_invert_plain_matches_oracle(chip, orc):
+    dims = (4, 4, 4, 8)
+    print("invert_Qtm kappa=0.14", flush=True)
+    g = chip.gauge_random(dims, 6, 0.3)
+    kappa, mu = 0.14, 0.01          # BiCGStab on D needs kappa below critical on this field
+    ctx = chip.Context(dims, kappa, mu, BC, gauge=g)
+    L = orc.Lattice(dims, kappa, mu, BC)
+    rng = np.random.default_rng(8)
+    b = rand_spinor(rng, L.V, 3)
+    rc, x, st = ctx.invert_Qtm(b, 1e-24, 2000)
+    assert rc > 0 and st["converged"] == 3 and st["true_res2_max"] < 1.1e-24
+    for r in range(3):
+        it, xo, tr = L.invert_Qtm(g, b[r], 1e-24, 2000, lagged=False)
+        assert it > 0 and abs(it - rc) <= 3
+        assert rel_err(x[r], xo) < 1e-8
+    # the lagged mode returns the iterate before the last update (invert_Qtm.c:199-211)
+    rc2, x2, st2 = ctx.invert_Qtm(b, 1e-24, 2000, lagged=True)
+    assert rc2 == rc and st2["true_res2_max"] > st["true_res2_max"]
+
+
+def test_invert_failures(chip):
+    dims = (4, 4, 4, 4)
+    g = chip.gauge_random(dims, 6, 0.3)
+    ctx = chip.Context(dims, 0.14, 0.01, BC, gauge=g)
+    b = np.zeros((1, ctx.V, 24))
+    b[0, 0, 0] = 1.0
+    rc, x, st = ctx.invert_Qtm(b + 0.1, 1e-24, 2, check=False)
+    assert rc == chip.ERR_NOCONV and st["converged"] == 0
+    # an exact start vector converges in 0 iterations (invert_Qtm.c:142-147)
+    rc, x, _ = ctx.invert_Qtm(b + 0.1, 1e-24, 2000)
+    rc2, _, _ = ctx.invert_Qtm(b + 0.1, 1e-16, 2000, x0=x)
+    assert rc2 == 0
+    with pytest.raises(chip.CvchipError):
+        ctx.invert_Qtm(b, 1e-24, 100, prec=chip.PREC_F32, delta=2.0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+def test_invert_her_config_8(chip, orc, lat8, prec):
+    """8^4, kappa=0.163, mu=0.01, 12 point-source rhs, squared tolerance 1e-20."""
+    dims, g = lat8
+    print(f"invert_Qtm_her 8^4 prec={prec}", flush=True)
+    P = {"f64": chip.PREC_F64, "f32": chip.PREC_F32, "f16": chip.PREC_F16}[prec]
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=g)
+    V = ctx.V
+    b = np.stack([chip.point_source(V, 0, a // 3, a % 3) for a in range(12)])
+    rc, x, st = ctx.invert_Qtm_her(b, 1e-20, 5000, prec=P, delta=0.1)
+    assert rc > 0 and st["converged"] == 12
+    assert st["true_res2_max"] < 1e-20
+    L = orc.Lattice(dims, 0.163, 0.01, BC)
+    for a in (0, 7):
+        # solution check by the oracle operator: |D x - b|^2/|b|^2
+        res = L.Q_phi(g, x[a]) - b[a]
+        assert L.dot_re(res, res) < 1e-20
+    if prec == "f64":
+        it, xo, tr = L.invert_Qtm_her(g, b[4], 1e-24, 5000)
+        rc2, x2, _ = ctx.invert_Qtm_her(b[4:5], 1e-24, 5000)
+        assert abs(rc2 - it) <= 3
+        assert rel_err(x2[0], xo) < 1e-8
