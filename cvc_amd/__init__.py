@@ -228,13 +228,14 @@ class Context:
             raise CvchipError(rc, last_error())
         return rc, x, st.as_dict()
 
-    def invert_Qtm_her(self, b, sp, niter_max=1000, x0=None, prec=PREC_F64, delta=0.1, check=True):
+    def invert_Qtm_her(self, b, sp, niter_max=1000, x0=None, prec=PREC_F64, delta=0.1, lagged=False,
+                       check=True):
         b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, self.V, 24)
         n = b.shape[0]
         x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, order="C").reshape(b.shape)
         st = Stats()
         rc = lib().cvchip_invert_Qtm_her(self._h, _ptr(x), _ptr(b), n, float(sp), int(niter_max), int(prec),
-                                         float(delta), 0, C.byref(st))
+                                         float(delta), SOLVE_LAGGED if lagged else 0, C.byref(st))
         if check and rc < 0:
             raise CvchipError(rc, last_error())
         return rc, x, st.as_dict()

@@ -20,6 +20,7 @@ template <typename R>
 struct DParams {
   uint32_t V, VN, sX, sT;
   int N, T, LX, LY, LZ;
+  int partner;   // DOT_TS: dot with the separate field r2 instead of the input
   R k2, mutm;
   cplx<R> ph[4];
 };
@@ -147,7 +148,8 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     }
     out.store(acc, i);
     if constexpr (DM == DOT_TS) {
-      // <out, in> = sum conj(out) in ; <out, out>
+      // <out, in> = sum conj(out) in ; <out, out>   (or <out, r2> when a partner is given)
+      if (P.partner) r2.load(own, i);
       R a = 0, b = 0, n = 0;
 #pragma unroll
       for (int c = 0; c < 12; c++) {
@@ -209,6 +211,7 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   using R = typename IOI::real;
   const int N = in.nrhs;
   DParams<R> P = make_params<R>(c, N, mutm);
+  P.partner = (dm == DOT_TS && r2 != nullptr) ? 1 : 0;
   const int B = block_for(N);
   const unsigned nblk = (unsigned)((P.VN + B - 1) / B);
   double *part = nullptr;

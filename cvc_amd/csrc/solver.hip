@@ -457,7 +457,8 @@ struct Op {
   int apply(Field &out, const Field &in, DotMode dm, const Field *r2, int *nb) {
     if (!her) return dslash(c, out, in, c->mu, 0, dm, r2, nb);
     CVC_TRY(dslash(c, *aux, in, -c->mu, 1));
-    return dslash(c, out, *aux, c->mu, 1, dm, r2, nb);
+    // the fused <t,s> must pair the output with the operator's input, not with aux
+    return dslash(c, out, *aux, c->mu, 1, dm, dm == DOT_TS ? &in : r2, nb);
   }
 };
 
@@ -738,7 +739,7 @@ int invert_her_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
     if (!rc && hipMemcpyAsync(y.data, x.data, x.bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) rc = CVCHIP_ERR_HIP;
     if (!rc) {
       hipLaunchKernelGGL(k_g5, dim3((VN + 255) / 256), dim3(256), 0, c->stream, (double2 *)g5b.data, VN);
-      if (prec == CVCHIP_PREC_F64) rc = bicgstab64(c, y, g5b, sp, niter_max, flags | CVCHIP_SOLVE_LAGGED, stats, 1);
+      if (prec == CVCHIP_PREC_F64) rc = bicgstab64(c, y, g5b, sp, niter_max, flags, stats, 1);
       else if (!(delta > 0. && delta < 1.)) {
         set_error("reliable-update factor delta must lie in (0,1), got %g", delta);
         rc = CVCHIP_ERR_ARG;

@@ -96,8 +96,9 @@ int cvchip_invert_Qtm(cvchip_ctx *ctx, double *x, const double *b, int nrhs, dou
 /* invert_Qtm_her (invert_Qtm.c:276-478): BiCGStab on gamma5 D(mu) gamma5 D(-mu)
  * y = gamma5 b (= D(-mu)^dagger D(-mu) y, Hermitian positive definite),
  * returning x = gamma5 D(-mu) y, which solves D(mu) x = b.  x is the initial
- * guess for y on entry, as in the reference.  prec / delta as for
- * cvchip_invert_Qtm (the mixed solver runs on the same Hermitian system). */
+ * guess for y on entry, as in the reference.  prec / delta / flags as for
+ * cvchip_invert_Qtm (the mixed solver runs on the same Hermitian system; the
+ * reference returns the lagged iterate here too: CVCHIP_SOLVE_LAGGED). */
 int cvchip_invert_Qtm_her(cvchip_ctx *ctx, double *x, const double *b, int nrhs, double sp,
                           int niter_max, int prec, double delta, unsigned flags,
                           cvchip_stats *stats);
@@ -152,9 +153,13 @@ int cvchip_wi(cvchip_ctx *ctx, double *wi, const double *conn);
 /* Momentum-space polarisation tensor (no reference counterpart; the reference
  * writes Pi(x) only, cvc_exact2_xspace.c:876-882):
  *   Pi_{mu nu}(q) = sum_x exp(-i q.(x - y)) exp(-i (q_mu - q_nu)/2) conn_{mu nu}(x)
- *                   + delta_{mu nu} * contact_mu * (-1/2)... see cvchip_pi_q notes in
- *   the implementation; q_mu = 2 pi n_mu / L_mu with n in [0, L_mu).
- * piq: double[V][16][2] in the site order of n. */
+ *                   - delta_{mu nu} contact_nu,
+ * q_mu = 2 pi n_mu / L_mu, n_mu in [0, L_mu), extents up to 64.  The half-link
+ * phases and the contact term make it transverse exactly,
+ * sum_mu 2 sin(q_mu/2) Pi_{mu nu}(q) = 0, which is the momentum-space form of
+ * the position-space identity
+ * sum_mu conn_{mu nu}(x) - conn_{mu nu}(x-mu) = contact_nu (delta_{x,y} - delta_{x,y+nu}).
+ * piq: double[V][16][2] with n in the lexicographic site order. */
 int cvchip_pi_q(cvchip_ctx *ctx, double *piq, const double *conn, const double contact[8],
                 const int src[4]);
 
