@@ -25,6 +25,7 @@ EXPORTS = [
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
     "cvchip_field_destroy", "cvchip_field_upload", "cvchip_field_download", "cvchip_field_copy",
+    "cvchip_field_fill_random",
     "cvchip_field_dslash",
     "cvchip_field_dslash_timed", "cvchip_field_invert", "cvchip_field_invert_her", "cvchip_cvc_contract",
     "cvchip_field_cvc_contract", "cvchip_wi", "cvchip_pi_q", "cvchip_cvc_source", "cvchip_gauge_random",
@@ -75,6 +76,7 @@ def lib():
         L.cvchip_field_upload.argtypes = [_vp, _vp, _dp, C.c_int, C.c_int]
         L.cvchip_field_download.argtypes = [_vp, _vp, _dp, C.c_int, C.c_int]
         L.cvchip_field_copy.argtypes = [_vp, _vp, _vp]
+        L.cvchip_field_fill_random.argtypes = [_vp, _vp, C.c_uint64]
         L.cvchip_field_dslash.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int]
         L.cvchip_field_dslash_timed.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int, _dp]
         L.cvchip_field_invert.argtypes = [_vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_double,
@@ -142,6 +144,9 @@ class Field:
         out = np.empty((n, self.ctx.V, 24))
         _check(lib().cvchip_field_download(self.ctx._h, self._h, _ptr(out), int(rhs0), int(n)))
         return out
+
+    def fill_random(self, seed: int):
+        _check(lib().cvchip_field_fill_random(self.ctx._h, self._h, C.c_uint64(int(seed))))
 
     def copy_from(self, src: "Field"):
         _check(lib().cvchip_field_copy(self.ctx._h, self._h, src._h))

@@ -57,6 +57,10 @@ int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec) {
     set_error("unknown precision %d", prec);
     return CVCHIP_ERR_ARG;
   }
+  if ((uint64_t)c->lat.V * (uint64_t)nrhs >= ((uint64_t)1 << 32) - 1024) {
+    set_error("volume x nrhs = %lld exceeds the 32-bit site-rhs index", (long long)(c->lat.V * nrhs));
+    return CVCHIP_ERR_ARG;
+  }
   f.nrhs = nrhs;
   f.prec = prec;
   f.V = c->lat.V;
@@ -122,11 +126,41 @@ __global__ void k_gauge_convert(double2 *__restrict__ g18, double2 *__restrict__
   }
 }
 
+// deterministic fill: each real = uniform(-1, 1) from a 64-bit mix of (seed, index)
+__device__ __forceinline__ double mix_uniform(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+__global__ void k_fill_random(double2 *__restrict__ f, uint64_t n, uint64_t seed) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  f[i] = make_double2(mix_uniform(seed * 0x100000001B3ull + 2 * i), mix_uniform(seed * 0x100000001B3ull + 2 * i + 1));
+}
+
 }  // namespace cvchip
 
 using namespace cvchip;
 
 // =================================================================== C ABI
+
+extern "C" int cvchip_field_fill_random(cvchip_ctx *c, cvchip_field *fh, uint64_t seed) {
+  if (!c || !fh) return CVCHIP_ERR_NULL;
+  Field &f = fh->f;
+  if (f.prec != CVCHIP_PREC_F64) {
+    set_error("random fill is defined for FP64 sets (convert with cvchip_field_copy)");
+    return CVCHIP_ERR_ARG;
+  }
+  const uint64_t n = (uint64_t)12 * f.V * f.nrhs;
+  hipLaunchKernelGGL(k_fill_random, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (double2 *)f.data,
+                     n, seed);
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
+}
 
 extern "C" const char *cvchip_last_error(void) { return g_err; }
 
@@ -142,7 +176,7 @@ extern "C" int cvchip_ctx_create(cvchip_ctx **ctx, const int dims[4], const doub
       return CVCHIP_ERR_ARG;
     }
   int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
-  if (V * 256 > (int64_t)INT32_MAX * 4) {
+  if (V >= ((int64_t)1 << 31)) {
     set_error("lattice volume %lld too large", (long long)V);
     return CVCHIP_ERR_ARG;
   }

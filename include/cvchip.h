@@ -110,6 +110,9 @@ int cvchip_field_create(cvchip_ctx *ctx, int nrhs, int prec, cvchip_field **f);
 int cvchip_field_destroy(cvchip_field *f);
 int cvchip_field_upload(cvchip_ctx *ctx, cvchip_field *f, const double *host, int rhs0, int n);
 int cvchip_field_download(cvchip_ctx *ctx, const cvchip_field *f, double *host, int rhs0, int n);
+/* Fill an FP64 set with deterministic uniform(-1,1) values from (seed, index);
+ * synthetic operator inputs that never leave HBM.  No reference counterpart. */
+int cvchip_field_fill_random(cvchip_ctx *ctx, cvchip_field *f, uint64_t seed);
 /* dst = src, converting precision (FP64 / FP32 / FP16+norm); equal nrhs. */
 int cvchip_field_copy(cvchip_ctx *ctx, cvchip_field *dst, const cvchip_field *src);
 /* out = D(mutm) in (gamma5_out: gamma5 D in) on all rhs of the set; precision
