@@ -114,7 +114,8 @@ def main():
             x = ctx.field(N)
             torch.cuda.synchronize()
             s0 = time.perf_counter()
-            rc, st = ctx.invert_her(x, b, 1e-20, 20000, prec=prec, delta=a.delta, check=False)
+            rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
+                                    delta=a.delta, check=False)
             torch.cuda.synchronize()
             el = time.perf_counter() - s0
             extra[f"solve_{name}_s"] = round(el, 3)

@@ -18,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcvchip.so")
+DROPIN_SRC = os.path.join(HERE, "dropin", "cvc_dropin.cc")
+DROPIN_OBJ = os.path.join(HERE, "dropin", "cvc_dropin.o")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -76,6 +78,13 @@ def build(verbose: bool = False) -> str:
     link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"]
     _run_all([link])
     shutil.move(tmp, LIB)
+    # the drop-in shim is linked by a reference program, not into libcvchip.so
+    _run_all([["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-c", DROPIN_SRC, "-o", DROPIN_OBJ]])
+    # a stand-in reference program that links the shim (used by the drop-in test)
+    drv = os.path.join(os.path.dirname(HERE), "tests", "dropin")
+    if os.path.exists(os.path.join(drv, "driver.cc")):
+        _run_all([["g++", "-O2", "-std=c++17", os.path.join(drv, "driver.cc"), DROPIN_OBJ, f"-L{HERE}", "-lcvchip",
+                   "-Wl,-rpath,$ORIGIN/../../cvc_amd", "-o", os.path.join(drv, "driver")]])
     return LIB
 
 

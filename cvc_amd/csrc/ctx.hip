@@ -5,7 +5,7 @@
 #include <string.h>
 #include <math.h>
 
-#include "cvchip_internal.h"
+#include "spinor_io.h"
 
 namespace cvchip {
 
@@ -90,7 +90,8 @@ __global__ void k_aos_to_field(cplx<R> *__restrict__ f, const double *__restrict
   int64_t s = sv % V;
   int r = (int)(sv / V);
   const double *p = aos + ((int64_t)r * V + s) * 24 + 2 * c;
-  f[((int64_t)c * V + s) * N + rhs0 + r] = mk<R>((R)p[0], (R)p[1]);
+  f[cplx_index(c, (uint32_t)(s * N + rhs0 + r), (uint32_t)(V * N), 16 / (int)sizeof(cplx<R>))] =
+      mk<R>((R)p[0], (R)p[1]);
 }
 
 template <typename R>
@@ -102,7 +103,7 @@ __global__ void k_field_to_aos(double *__restrict__ aos, const cplx<R> *__restri
   int64_t sv = i / 12;
   int64_t s = sv % V;
   int r = (int)(sv / V);
-  cplx<R> v = f[((int64_t)c * V + s) * N + rhs0 + r];
+  cplx<R> v = f[cplx_index(c, (uint32_t)(s * N + rhs0 + r), (uint32_t)(V * N), 16 / (int)sizeof(cplx<R>))];
   double *p = aos + ((int64_t)r * V + s) * 24 + 2 * c;
   p[0] = (double)v.x;
   p[1] = (double)v.y;

@@ -151,6 +151,9 @@ enum DotMode { DOT_NONE = 0, DOT_TS = 1, DOT_R2 = 2 };
 int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_out,
            DotMode dm = DOT_NONE, const Field *r2 = nullptr, int *nblocks_out = nullptr);
 
+// number of blocks (= reduction partial slots) of an operator launch on N rhs
+unsigned dslash_blocks(const cvchip_ctx *c, int N);
+
 // block-size convention shared by all per-rhs reductions
 inline int block_for(int nrhs) {
   int s = 256 / nrhs;

@@ -21,19 +21,26 @@ struct DParams {
   uint32_t V, VN, sX, sT;
   int N, T, LX, LY, LZ;
   int partner;   // DOT_TS: dot with the separate field r2 instead of the input
+  // traversal: the lattice is cut into chunks of csz consecutive sites (one
+  // eighth of a time slice when possible), bpc blocks per chunk; with xcd set
+  // the 8 XCDs take the 8 chunks of a time slice side by side, so the whole
+  // GPU advances through the slices together (t neighbours stay in the MALL)
+  // while each XCD's L2 holds the x/y/z neighbours of its own chunk
+  uint32_t csz, bpc;
+  int xcd;
   R k2, mutm;
   cplx<R> ph[4];
 };
 
+// u[k] = base[k * stride] for the stored entries (9, or the first two rows)
 template <typename R, int RECON>
-__device__ __forceinline__ void load_link(cplx<R> u[9], const cplx<R> *__restrict__ U, int mu,
-                                          uint32_t s, uint32_t V) {
+__device__ __forceinline__ void load_link(cplx<R> u[9], const cplx<R> *__restrict__ base, uint32_t stride) {
   if constexpr (RECON == 18) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = U[(size_t)(mu * 9 + k) * V + s];
+    for (int k = 0; k < 9; k++) u[k] = base[(size_t)k * stride];
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; k++) u[k] = U[(size_t)(mu * 6 + k) * V + s];
+    for (int k = 0; k < 6; k++) u[k] = base[(size_t)k * stride];
     // SU(3): row2 = conj(row0 x row1)
     cplx<R> a = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
     cplx<R> b = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
@@ -88,36 +95,92 @@ __device__ __forceinline__ void hop(cplx<R> acc[12], const cplx<R> psi[12], cons
   }
 }
 
-template <typename IO, int RECON, int MU>
+// Links of a block staged in LDS: [8 hops][KL entries][LDS_SITES], hop 2*mu is
+// U_mu(x - mu) (backward), 2*mu + 1 is U_mu(x) (forward).
+constexpr int LDS_SITES = 32;
+
+template <typename IO, int RECON, int MU, bool LDSL>
 __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const IO &in,
                                          const cplx<typename IO::real> *__restrict__ U,
-                                         const DParams<typename IO::real> &P, uint32_t s, uint32_t r,
-                                         int coord, int L, uint32_t stride) {
+                                         const cplx<typename IO::real> *__restrict__ Ul,
+                                         const DParams<typename IO::real> &P, uint32_t s, uint32_t sl,
+                                         uint32_t r, int coord, int L, uint32_t stride) {
   using R = typename IO::real;
+  constexpr int KL = RECON == 18 ? 9 : 6;
   cplx<R> psi[12], u[9];
   const uint32_t sm = coord == 0 ? s + (uint32_t)(L - 1) * stride : s - stride;
   const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
   // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
   in.load(psi, sm * P.N + r);
-  load_link<R, RECON>(u, U, MU, sm, P.V);
+  if constexpr (LDSL) load_link<R, RECON>(u, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES);
+  else load_link<R, RECON>(u, U + (size_t)MU * KL * P.V + sm, P.V);
   hop<R, MU, false>(acc, psi, u, P.ph[MU]);
   // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
   in.load(psi, sp * P.N + r);
-  load_link<R, RECON>(u, U, MU, s, P.V);
+  if constexpr (LDSL) load_link<R, RECON>(u, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES);
+  else load_link<R, RECON>(u, U + (size_t)MU * KL * P.V + s, P.V);
   hop<R, MU, true>(acc, psi, u, P.ph[MU]);
 }
 
-template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM>
-__global__ void __launch_bounds__(256)
+// Minimum resident waves per SIMD the register allocation must allow (the
+// compiler otherwise hoists every neighbour load and ends at one wave).
+#ifndef CVC_DSLASH_WAVES64
+#define CVC_DSLASH_WAVES64 1
+#endif
+#ifndef CVC_DSLASH_WAVES32
+#define CVC_DSLASH_WAVES32 3
+#endif
+template <typename R> struct DslashWaves {
+  static constexpr int value = sizeof(R) == 8 ? CVC_DSLASH_WAVES64 : CVC_DSLASH_WAVES32;
+};
+
+// LDSL: the block's sites share their links through LDS (N >= 8, at most
+// LDS_SITES sites per block): every link entry crosses the texture path once
+// per block instead of once per rhs lane.
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL>
+__global__ void __launch_bounds__(256, DslashWaves<typename IOI::real>::value)
 k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParams<typename IOI::real> P,
          IOI r2, double *__restrict__ part) {
   using R = typename IOI::real;
   using C = cplx<R>;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int KL = RECON == 18 ? 9 : 6;
+  const uint32_t S = blockDim.x / (uint32_t)P.N;
+  uint32_t chunk, blk;
+  if (P.xcd) {
+    const uint32_t j = blockIdx.x >> 3;
+    chunk = (j / P.bpc) * kXcds + (blockIdx.x & (kXcds - 1));
+    blk = j % P.bpc;
+  } else {
+    chunk = blockIdx.x / P.bpc;
+    blk = blockIdx.x % P.bpc;
+  }
+  const uint32_t s0 = chunk * P.csz + blk * S;
+  const uint32_t s_end = min(s0 + S, (chunk + 1) * P.csz);
+  const uint32_t r = threadIdx.x % (uint32_t)P.N;
+  const uint32_t s = s0 + threadIdx.x / (uint32_t)P.N;
+  const uint32_t i = s * (uint32_t)P.N + r;
   double dv[3] = {0., 0., 0.};
-  if (i < P.VN) {
-    const uint32_t s = i / (uint32_t)P.N;
-    const uint32_t r = i - s * (uint32_t)P.N;
+  __shared__ C Ul[LDSL ? 8 * KL * LDS_SITES : 1];
+  if constexpr (LDSL) {
+    const uint32_t tot = 8u * KL * S;
+    for (uint32_t e = threadIdx.x; e < tot; e += blockDim.x) {
+      const uint32_t hopi = e / (KL * S), rem = e - hopi * (KL * S);
+      const uint32_t k = rem / S, sl = rem - k * S;
+      const uint32_t site = s0 + sl;
+      if (site >= s_end) continue;
+      const int mu = (int)(hopi >> 1);
+      uint32_t ls = site;
+      if ((hopi & 1u) == 0) {          // backward hop: link at site - mu
+        const uint32_t str = mu == 0 ? P.sT : mu == 1 ? P.sX : mu == 2 ? (uint32_t)P.LZ : 1u;
+        const uint32_t L = (uint32_t)(mu == 0 ? P.T : mu == 1 ? P.LX : mu == 2 ? P.LY : P.LZ);
+        const uint32_t crd = (site / str) % L;
+        ls = crd == 0 ? site + (L - 1) * str : site - str;
+      }
+      Ul[(hopi * KL + k) * LDS_SITES + sl] = U[((size_t)mu * KL + k) * P.V + ls];
+    }
+    __syncthreads();
+  }
+  if (s < s_end) {
     const uint32_t t = s / P.sT;
     uint32_t rem = s - t * P.sT;
     const uint32_t x = rem / P.sX;
@@ -128,10 +191,11 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     C acc[12];
 #pragma unroll
     for (int c = 0; c < 12; c++) acc[c] = mk<R>(0, 0);
-    hop_pair<IOI, RECON, 0>(acc, in, U, P, s, r, (int)t, P.T, P.sT);
-    hop_pair<IOI, RECON, 1>(acc, in, U, P, s, r, (int)x, P.LX, P.sX);
-    hop_pair<IOI, RECON, 2>(acc, in, U, P, s, r, (int)y, P.LY, (uint32_t)P.LZ);
-    hop_pair<IOI, RECON, 3>(acc, in, U, P, s, r, (int)z, P.LZ, 1u);
+    const uint32_t sl = s - s0;
+    hop_pair<IOI, RECON, 0, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)t, P.T, P.sT);
+    hop_pair<IOI, RECON, 1, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)x, P.LX, P.sX);
+    hop_pair<IOI, RECON, 2, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)y, P.LY, (uint32_t)P.LZ);
+    hop_pair<IOI, RECON, 3, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)z, P.LZ, 1u);
 
     C own[12];
     in.load(own, i);
@@ -205,6 +269,26 @@ template <> IOHalf make_io<IOHalf>(const Field &f) {
   return IOHalf{(__half2 *)f.data, f.norm, (uint32_t)(f.V * f.nrhs)};
 }
 
+struct Traversal {
+  uint32_t csz, bpc, nblk;
+  int xcd;
+};
+
+static Traversal traversal(const cvchip_ctx *c, int N) {
+  const Lattice &L = c->lat;
+  const uint32_t V = (uint32_t)L.V, slice = (uint32_t)(L.LX * L.LY * L.LZ);
+  const uint32_t S = (uint32_t)(block_for(N) / N);
+  Traversal t;
+  if (slice % kXcds == 0) { t.csz = slice / kXcds; t.xcd = 1; }
+  else if (V % kXcds == 0) { t.csz = V / kXcds; t.xcd = 1; }
+  else { t.csz = V; t.xcd = 0; }
+  t.bpc = (t.csz + S - 1) / S;
+  t.nblk = (V / t.csz) * t.bpc;
+  return t;
+}
+
+unsigned dslash_blocks(const cvchip_ctx *c, int N) { return traversal(c, N).nblk; }
+
 template <typename IOI, typename IOO, int RECON>
 static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g5, DotMode dm,
                   const Field *r2, int *nblk_out) {
@@ -213,7 +297,11 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   DParams<R> P = make_params<R>(c, N, mutm);
   P.partner = (dm == DOT_TS && r2 != nullptr) ? 1 : 0;
   const int B = block_for(N);
-  const unsigned nblk = (unsigned)((P.VN + B - 1) / B);
+  const Traversal tr = traversal(c, N);
+  P.csz = tr.csz;
+  P.bpc = tr.bpc;
+  P.xcd = tr.xcd;
+  const unsigned nblk = tr.nblk;
   double *part = nullptr;
   if (dm != DOT_NONE) {
     CVC_TRY(ensure_red(c, sizeof(double) * ((size_t)nblk * 3 * N + 64 * 256)));
@@ -226,8 +314,12 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   IOI ii = make_io<IOI>(in);
   IOI rr = r2 ? make_io<IOI>(*r2) : ii;
   dim3 g(nblk), b(B);
-#define CVC_LAUNCH(G5, DMV) \
-  hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV>), g, b, 0, c->stream, o, ii, U, P, rr, part)
+  const bool ldsl = sizeof(R) == 4 && B / N <= LDS_SITES;
+#define CVC_LAUNCH(G5, DMV)                                                                               \
+  do {                                                                                                    \
+    if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
+    else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
+  } while (0)
   if (g5) {
     if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);
     else if (dm == DOT_TS) CVC_LAUNCH(true, DOT_TS);

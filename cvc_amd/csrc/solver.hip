@@ -24,6 +24,7 @@
 // solution into the FP64 x at each update.  Convergence is always decided on
 // the FP64 true residual.
 #include <stdio.h>
+#include <algorithm>
 #include <chrono>
 #include <vector>
 
@@ -36,7 +37,7 @@ struct RhsState {
   double normb, norm, tol2;
   double rmax;        // mixed: |r|^2 at the last reliable update
   double true_res2;
-  int done, final_upd, iter, upd;
+  int done, final_upd, iter, upd, last_upd;
 };
 
 // ------------------------------------------------------------ vector kernels
@@ -248,6 +249,7 @@ __global__ void k_sc_init(RhsState *st, const double *res, int N, double sp, int
   S.rmax = S.norm;
   S.final_upd = 0;
   S.upd = 0;
+  S.last_upd = 0;
   S.iter = -1;
   S.done = (S.norm <= S.tol2) ? 1 : 0;       // invert_Qtm.c:142-147: return 0
   if (S.done) S.iter = 0;
@@ -288,6 +290,8 @@ __global__ void k_sc_omega(RhsState *st, const double *res, int N) {
   S.omega[1] = res[N + r] * inv;
 }
 
+constexpr int kMaxGap = 48;
+
 // after r1: convergence test (or reliable-update trigger), beta, r0rn
 __global__ void k_sc_beta(RhsState *st, const double *res, int N, int it, int lagged, int mixed,
                           double delta2) {
@@ -306,8 +310,9 @@ __global__ void k_sc_beta(RhsState *st, const double *res, int N, int it, int la
     }
   } else {
     // reliable update when the iterated residual fell by delta since the last
-    // update, or reached the tolerance (then decided in FP64)
-    S.upd = (S.norm <= delta2 * S.rmax || S.norm <= S.tol2) ? 1 : 0;
+    // update, reached the tolerance (then decided in FP64), or has not done so
+    // for kMaxGap iterations (a low-precision recurrence can stall above delta)
+    S.upd = (S.norm <= delta2 * S.rmax || S.norm <= S.tol2 || it - S.last_upd >= kMaxGap) ? 1 : 0;
   }
   double w2[2], w3[2];
   cdiv(w2, w, S.r0rn);
@@ -328,6 +333,7 @@ __global__ void k_sc_reliable(RhsState *st, const double *res, int N, int it) {
   if (!S.upd) return;
   S.norm = res[r];
   S.rmax = S.norm;
+  S.last_upd = it;
   if (S.norm <= S.tol2) {
     S.done = 1;
     S.iter = it;
@@ -394,7 +400,9 @@ static int solve_setup(cvchip_ctx *c, Solve &S, int N) {
   S.VN = (uint32_t)(c->lat.V * N);
   S.B = block_for(N);
   S.nblk = (S.VN + S.B - 1) / S.B;
-  CVC_TRY(ensure_red(c, sizeof(double) * ((size_t)S.nblk * 3 * N + 64 * 256)));
+  // room for the operator's partials too (its traversal has its own block count)
+  const size_t nb_max = std::max<size_t>(S.nblk, dslash_blocks(c, N));
+  CVC_TRY(ensure_red(c, sizeof(double) * (nb_max * 3 * N + 64 * 256)));
   CVC_HIP(hipMalloc(&S.st, sizeof(RhsState) * N));
   CVC_HIP(hipMalloc(&S.res, sizeof(double) * 4 * N));
   CVC_HIP(hipHostMalloc(&S.hst, sizeof(RhsState) * N, hipHostMallocDefault));

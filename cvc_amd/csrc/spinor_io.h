@@ -6,11 +6,26 @@
 
 namespace cvchip {
 
-// Plain storage: cplx<R>[12][V][N]; idx = site*N + rhs; planes VN apart.
-template <typename R>
-struct IOPlain {
-  using real = R;
-  using C = cplx<R>;
+// Every load and store of a (site, rhs) moves 16 bytes per lane (the widest
+// vector memory access), because the texture addresser's cost is per lane
+// and per instruction rather than per byte.  Components are therefore grouped
+// K = 16 B / sizeof(complex) at a time:
+//   FP64: K = 1, 12 planes of double2       element (c, idx) at  c*VN + idx
+//   FP32: K = 2,  6 planes of float4        element (c, idx) at (c/2)*VN + idx, half c%2
+//   FP16: K = 4,  3 planes of 4 x half2     element (c, idx) at (c/4)*VN + idx, quarter c%4
+// with idx = site*N + rhs and VN = V*N.  cplx_index() gives the position in
+// units of one complex number.
+__host__ __device__ __forceinline__ size_t cplx_index(int c, uint32_t idx, uint32_t VN, int K) {
+  return ((size_t)(c / K) * VN + idx) * K + (c % K);
+}
+
+template <typename R> struct IOPlain;
+
+template <>
+struct IOPlain<double> {
+  using real = double;
+  using C = double2;
+  static constexpr int K = 1;
   C *__restrict__ p;
   uint32_t VN;
   __device__ __forceinline__ void load(C v[12], uint32_t idx) const {
@@ -23,20 +38,52 @@ struct IOPlain {
   }
 };
 
+template <>
+struct IOPlain<float> {
+  using real = float;
+  using C = float2;
+  static constexpr int K = 2;
+  C *__restrict__ p;   // viewed as float4 planes
+  uint32_t VN;
+  __device__ __forceinline__ void load(C v[12], uint32_t idx) const {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const float4 f = q[(size_t)k * VN + idx];
+      v[2 * k] = make_float2(f.x, f.y);
+      v[2 * k + 1] = make_float2(f.z, f.w);
+    }
+  }
+  __device__ __forceinline__ void store(const C v[12], uint32_t idx) const {
+    float4 *q = reinterpret_cast<float4 *>(p);
+#pragma unroll
+    for (int k = 0; k < 6; k++) q[(size_t)k * VN + idx] = make_float4(v[2 * k].x, v[2 * k].y, v[2 * k + 1].x, v[2 * k + 1].y);
+  }
+};
+
 // Half storage with one float scale per (site, rhs): value = half * norm[idx].
 // Arithmetic in FP32.  The scale is the largest |component| of the site.
 struct IOHalf {
   using real = float;
   using C = float2;
-  __half2 *__restrict__ p;
+  static constexpr int K = 4;
+  __half2 *__restrict__ p;   // viewed as uint4 planes (4 x half2)
   float *__restrict__ nrm;
   uint32_t VN;
   __device__ __forceinline__ void load(C v[12], uint32_t idx) const {
     const float n = nrm[idx];
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
 #pragma unroll
-    for (int c = 0; c < 12; c++) {
-      float2 f = __half22float2(p[(size_t)c * VN + idx]);
-      v[c] = make_float2(f.x * n, f.y * n);
+    for (int k = 0; k < 3; k++) {
+      const uint4 w = q[(size_t)k * VN + idx];
+      const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        __half2 h;
+        __builtin_memcpy(&h, &u[j], 4);
+        const float2 f = __half22float2(h);
+        v[4 * k + j] = make_float2(f.x * n, f.y * n);
+      }
     }
   }
   __device__ __forceinline__ void store(const C v[12], uint32_t idx) const {
@@ -45,17 +92,28 @@ struct IOHalf {
     for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(v[c].x), fabsf(v[c].y)));
     const float inv = m > 0.f ? 1.0f / m : 0.f;
     nrm[idx] = m;
+    uint4 *q = reinterpret_cast<uint4 *>(p);
 #pragma unroll
-    for (int c = 0; c < 12; c++)
-      p[(size_t)c * VN + idx] = __floats2half2_rn(v[c].x * inv, v[c].y * inv);
+    for (int k = 0; k < 3; k++) {
+      uint32_t u[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const __half2 h = __floats2half2_rn(v[4 * k + j].x * inv, v[4 * k + j].y * inv);
+        __builtin_memcpy(&u[j], &h, 4);
+      }
+      q[(size_t)k * VN + idx] = make_uint4(u[0], u[1], u[2], u[3]);
+    }
   }
 };
 
 // Sums Q per-thread values over the threads of a block that share a
 // right-hand side (thread t holds rhs t % N; blockDim.x is a multiple of N)
-// with a fixed-order tree, and writes part[(block*Q + q)*N + rhs].
+// with a fixed-order tree, and writes part[(slot*Q + q)*N + rhs] (slot defaults
+// to the block index).
 template <int Q>
-__device__ __forceinline__ void block_reduce_store(const double v[Q], double *__restrict__ part, int N) {
+__device__ __forceinline__ void block_reduce_store(const double v[Q], double *__restrict__ part, int N,
+                                                   uint32_t slot = 0xffffffffu) {
+  if (slot == 0xffffffffu) slot = blockIdx.x;
   __shared__ double sh[Q][256];
   const int tid = threadIdx.x;
   const int B = blockDim.x;
@@ -75,7 +133,7 @@ __device__ __forceinline__ void block_reduce_store(const double v[Q], double *__
   }
   if (tid < N) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) part[((size_t)blockIdx.x * Q + q) * N + tid] = sh[q][tid];
+    for (int q = 0; q < Q; q++) part[((size_t)slot * Q + q) * N + tid] = sh[q][tid];
   }
 }
 
@@ -85,5 +143,8 @@ __global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int
                                   double *__restrict__ res);
 
 int reduce_partials(cvchip_ctx *c, const double *part, int nblk, int Q, int N, double *res);
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8).
+constexpr unsigned kXcds = 8;
 
 }  // namespace cvchip

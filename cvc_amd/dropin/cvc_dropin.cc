@@ -1,0 +1,184 @@
+// Drop-in replacement of the reference's operator and solver translation
+// units (Q_phi.c / invert_Qtm.c) for the twisted-mass path.  A reference
+// program links this file and libcvchip.so instead of Q_phi.o and
+// invert_Qtm.o; the functions keep the reference's names, C++ linkage in
+// namespace cvc, argument meaning, return codes and stdout lines, and read the
+// same globals (global.h:91-154), which the program defines as before.
+//
+// Replaced entry points (reference file:line):
+//   void Q_phi_tbc(double *xi, double *phi)                  Q_phi.h:10, Q_phi.c:38-188
+//   void Q_phi(double *xi, double *phi, const double mutm)   Q_phi.h:28, Q_phi.c:1092-1242
+//   void Qf5(double *xi, double *phi, double mutm)           Q_phi.h:26, Q_phi.c:789-934
+//   void g5_phi(double *phi)                                 Q_phi.h:37, Q_phi.c:350-385
+//   void spinor_scalar_product_co(complex*, double*, double*, int)   invert_Qtm.h:8, invert_Qtm.c:28-58
+//   void spinor_scalar_product_re(double*, double*, double*, int)    invert_Qtm.h:9, invert_Qtm.c:60-84
+//   int  invert_Qtm(double *xi, double *phi, int kwork)              invert_Qtm.h:10, invert_Qtm.c:86-270
+//   int  invert_Qtm_her(double *xi, double *phi, int kwork)          invert_Qtm.h:11, invert_Qtm.c:276-478
+//
+// Differences a caller can observe: the operator needs no halo exchange (one
+// GPU holds the whole lattice, so xchange_field() is not needed around it);
+// reductions are tree-ordered, so the last digits of dot products and the
+// Krylov trajectory differ from the sequential reference sums.  The solvers
+// keep the reference's lagged return iterate (invert_Qtm.c:199-211).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "cvchip.h"
+
+namespace cvc {
+
+// the reference's complex type (cvc_complex.h:16-18): same layout and name
+typedef struct {
+  double re, im;
+} complex;
+
+// globals defined by the reference program's MAIN_PROGRAM unit (global.h:91-154)
+extern int T, LX, LY, LZ, VOLUME, T_global;
+extern double **g_spinor_field;
+extern double *g_gauge_field;
+extern double g_kappa, g_mu;
+extern int g_cart_id;
+extern double BCangle[4];
+extern double solver_precision;
+extern int niter_max;
+
+}  // namespace cvc
+
+namespace {
+
+cvchip_ctx *g_ctx = nullptr;
+int g_dims[4] = {0, 0, 0, 0};
+double g_bc[4] = {0, 0, 0, 0};
+const double *g_gauge_seen = nullptr;
+double g_gauge_sum = 0.;
+double g_kappa_seen = 0., g_mu_seen = 0.;
+
+void die(int rc, const char *where) {
+  fprintf(stderr, "[cvc_dropin] %s failed (%d): %s\n", where, rc, cvchip_last_error());
+  exit(1);
+}
+
+double gauge_checksum(const double *g, long n) {
+  double s = 0.;
+  for (long i = 0; i < n; i += 97) s += g[i] * (double)((i % 13) + 1);
+  return s;
+}
+
+// Context for the current globals; (re)uploads the gauge field when the
+// pointer or a strided checksum of it changed since the last call.
+cvchip_ctx *ctx() {
+  using namespace cvc;
+  const int dims[4] = {T, LX, LY, LZ};
+  bool fresh = !g_ctx || memcmp(dims, g_dims, sizeof(dims)) != 0 || memcmp(BCangle, g_bc, sizeof(g_bc)) != 0;
+  if (fresh) {
+    if (g_ctx) cvchip_ctx_destroy(g_ctx);
+    g_ctx = nullptr;
+    int rc = cvchip_ctx_create(&g_ctx, dims, BCangle, g_kappa, g_mu);
+    if (rc) die(rc, "cvchip_ctx_create");
+    memcpy(g_dims, dims, sizeof(dims));
+    memcpy(g_bc, BCangle, sizeof(g_bc));
+    g_gauge_seen = nullptr;
+    g_kappa_seen = g_kappa;
+    g_mu_seen = g_mu;
+  }
+  if (g_kappa != g_kappa_seen || g_mu != g_mu_seen) {
+    int rc = cvchip_ctx_set_params(g_ctx, g_kappa, g_mu);
+    if (rc) die(rc, "cvchip_ctx_set_params");
+    g_kappa_seen = g_kappa;
+    g_mu_seen = g_mu;
+  }
+  const long n = 72L * VOLUME;
+  const double sum = g_gauge_field ? gauge_checksum(g_gauge_field, n) : 0.;
+  if (g_gauge_field && (g_gauge_field != g_gauge_seen || sum != g_gauge_sum)) {
+    int rc = cvchip_set_gauge(g_ctx, g_gauge_field);
+    if (rc) die(rc, "cvchip_set_gauge");
+    g_gauge_seen = g_gauge_field;
+    g_gauge_sum = sum;
+  }
+  return g_ctx;
+}
+
+}  // namespace
+
+namespace cvc {
+
+void Q_phi_tbc(double *xi, double *phi) {
+  int rc = cvchip_Q_phi(ctx(), xi, phi, g_mu, 0);
+  if (rc) die(rc, "Q_phi_tbc");
+}
+
+void Q_phi(double *xi, double *phi, const double mutm) {
+  int rc = cvchip_Q_phi(ctx(), xi, phi, mutm, 0);
+  if (rc) die(rc, "Q_phi");
+}
+
+void Qf5(double *xi, double *phi, double mutm) {
+  int rc = cvchip_Q_phi(ctx(), xi, phi, mutm, 1);
+  if (rc) die(rc, "Qf5");
+}
+
+// gamma5 = diag(1,1,-1,-1) in the reference basis: negate spins 2 and 3
+void g5_phi(double *phi) {
+  for (long ix = 0; ix < (long)VOLUME; ix++)
+    for (int k = 12; k < 24; k++) phi[24 * ix + k] = -phi[24 * ix + k];
+}
+
+void spinor_scalar_product_co(complex *w, double *xi, double *phi, int V) {
+  if (V != VOLUME) {
+    fprintf(stderr, "[cvc_dropin] scalar product over %d sites, lattice has %d\n", V, VOLUME);
+    exit(1);
+  }
+  double r[2];
+  int rc = cvchip_scalar_product_co(ctx(), r, xi, phi);
+  if (rc) die(rc, "spinor_scalar_product_co");
+  w->re = r[0];
+  w->im = r[1];
+}
+
+void spinor_scalar_product_re(double *r, double *xi, double *phi, int V) {
+  if (V != VOLUME) {
+    fprintf(stderr, "[cvc_dropin] scalar product over %d sites, lattice has %d\n", V, VOLUME);
+    exit(1);
+  }
+  int rc = cvchip_scalar_product_re(ctx(), r, xi, phi);
+  if (rc) die(rc, "spinor_scalar_product_re");
+}
+
+static int solve(double *xi, double *phi, int kwork, bool her) {
+  // the reference tests r1, r2, s, t, p (g_spinor_field[kwork..kwork+4]), x and
+  // phi for NULL and returns -2 (invert_Qtm.c:112-113, 304-305); the device
+  // solver keeps its own workspace, the check is kept for the return code
+  if (!g_spinor_field || !xi || !phi) return -2;
+  for (int k = 0; k < 5; k++)
+    if (!g_spinor_field[kwork + k]) return -2;
+  double normb = 0.;
+  spinor_scalar_product_re(&normb, phi, phi, VOLUME);
+  if (g_cart_id == 0) fprintf(stdout, "# norm of r.-h. side: %e\n", normb);
+  cvchip_stats st;
+  memset(&st, 0, sizeof(st));
+  int rc = her ? cvchip_invert_Qtm_her(ctx(), xi, phi, 1, solver_precision, niter_max, CVCHIP_PREC_F64, 0.1,
+                                       CVCHIP_SOLVE_LAGGED, &st)
+               : cvchip_invert_Qtm(ctx(), xi, phi, 1, solver_precision, niter_max, CVCHIP_PREC_F64, 0.1,
+                                   CVCHIP_SOLVE_LAGGED, &st);
+  if (rc == CVCHIP_ERR_NOCONV) {
+    if (g_cart_id == 0)
+      fprintf(stdout, "# No convergence in BiCGStab; after %d steps relative residuum is %e\n", st.iterations,
+              st.true_res2_max);
+    return -3;
+  }
+  if (rc < 0) die(rc, her ? "invert_Qtm_her" : "invert_Qtm");
+  if (g_cart_id == 0 && rc > 0) {
+    fprintf(stdout, "# BiCGStab converged after %d steps\n", rc);
+    fprintf(stdout, "# true relative squared residuum is %e\n", st.true_res2_max);
+  }
+  if (g_cart_id == 0 && rc == 0) fprintf(stdout, "start spinor solves to requested precision\n");
+  return rc;
+}
+
+int invert_Qtm(double *xi, double *phi, int kwork) { return solve(xi, phi, kwork, false); }
+
+int invert_Qtm_her(double *xi, double *phi, int kwork) { return solve(xi, phi, kwork, true); }
+
+}  // namespace cvc
