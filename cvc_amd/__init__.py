@@ -28,7 +28,8 @@ EXPORTS = [
     "cvchip_field_fill_random",
     "cvchip_field_dslash",
     "cvchip_field_dslash_timed", "cvchip_field_invert", "cvchip_field_invert_her", "cvchip_cvc_contract",
-    "cvchip_field_cvc_contract", "cvchip_wi", "cvchip_pi_q", "cvchip_cvc_source", "cvchip_gauge_random",
+    "cvchip_field_cvc_contract", "cvchip_wi", "cvchip_pi_q", "cvchip_cvc_source", "cvchip_cvc_gradient",
+    "cvchip_gauge_random",
 ]
 
 
@@ -89,6 +90,8 @@ def lib():
         L.cvchip_pi_q.argtypes = [_vp, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.cvchip_cvc_source.argtypes = [_vp, _dp, _dp, C.POINTER(C.c_int), C.c_double, C.c_int, C.c_int,
                                         C.c_double, C.POINTER(Stats)]
+        L.cvchip_cvc_gradient.argtypes = [_vp, C.POINTER(C.c_int), C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.c_double, _dp, _dp, _dp, C.POINTER(Stats)]
         L.cvchip_gauge_random.argtypes = [C.POINTER(C.c_int), C.c_uint64, C.c_double, _dp]
         _lib = L
     return _lib
@@ -306,6 +309,20 @@ class Context:
         _check(lib().cvchip_pi_q(self._h, _ptr(out), _ptr(np.ascontiguousarray(conn)), _ptr(ct),
                                  (C.c_int * 4)(*src)))
         return out
+
+    def cvc_gradient(self, src, qmax=2.0, sp=1e-24, niter_max=5000, prec=PREC_F64, delta=0.1, force=True):
+        """L = sum_mu sum_{|q|<=qmax} Re Pi_mumu(q), dL/dmu and the link force F[V,4,3,3]
+        (dL/deps = tr(T F) for U -> exp(i eps T) U)."""
+        Lv, dmu = np.zeros(1), np.zeros(1)
+        F = np.empty((self.V, 4, 18)) if force else None
+        st = Stats()
+        _check(lib().cvchip_cvc_gradient(self._h, (C.c_int * 4)(*src), float(qmax), float(sp), int(niter_max),
+                                         int(prec), float(delta), _ptr(Lv), _ptr(dmu),
+                                         _ptr(F) if force else None, C.byref(st)))
+        Fc = None
+        if force:
+            Fc = (F[..., 0::2] + 1j * F[..., 1::2]).reshape(self.V, 4, 3, 3)
+        return float(Lv[0]), float(dmu[0]), Fc, st.as_dict()
 
     def cvc_source(self, src, sp, niter_max=1000, prec=PREC_F64, delta=0.1):
         conn = np.empty((self.V, 16, 2))

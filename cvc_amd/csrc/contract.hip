@@ -29,24 +29,12 @@
 #include <math.h>
 #include <vector>
 
-#include "spinor_io.h"
+#include "contract_common.h"
 
 namespace cvchip {
 
 int invert_her_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec,
                       double delta, unsigned flags, cvchip_stats *stats);
-
-struct SrcLinks {
-  double2 u[4][9];   // Us_nu = U_nu(y) * phase_nu
-};
-
-// gamma_nu gamma5 (nu = 0..3) and gamma5 as (column, unit value) per row.
-// gamma_nu gamma5 row a: gamma_nu[a][GCOL] * gamma5[GCOL][GCOL].
-__device__ __forceinline__ double2 unit_val(int code) {
-  return code == 0 ? make_double2(1, 0) : code == 1 ? make_double2(-1, 0)
-         : code == 2 ? make_double2(0, 1) : make_double2(0, -1);
-}
-__device__ __forceinline__ int g5sign(int spin) { return spin < 2 ? 1 : -1; }
 
 // Psi fields for one nu.  up/dn: 60-column sets (12*s + k); psi: 24-column set.
 __global__ void __launch_bounds__(256) k_psi(double2 *__restrict__ psi, const double2 *__restrict__ dn,
@@ -91,48 +79,6 @@ __global__ void __launch_bounds__(256) k_psi(double2 *__restrict__ psi, const do
     psi[(size_t)c * VP + (size_t)x * 24u + 12 + k] = p2[c];
   }
 }
-
-// v = gamma5 (gamma_mu -+ 1) w  (sgn = -1: Gm, +1: Gp), for runtime mu
-__device__ __forceinline__ void sink_vertex(double2 v[12], const double2 w[12], int mu, double sgn) {
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const int b = GCOL[mu][a];
-    const double2 e = unit_val(GVAL[mu][a]);
-    const double s5 = (double)g5sign(a);
-#pragma unroll
-    for (int cc = 0; cc < 3; cc++) {
-      double2 g = cmul(e, w[3 * b + cc]);
-      g.x += sgn * w[3 * a + cc].x;
-      g.y += sgn * w[3 * a + cc].y;
-      v[3 * a + cc] = make_double2(s5 * g.x, s5 * g.y);
-    }
-  }
-}
-
-__device__ __forceinline__ void su3_mul(double2 o[12], const double2 u[9], const double2 p[12], bool dag) {
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      double2 s = make_double2(0, 0);
-#pragma unroll
-      for (int j = 0; j < 3; j++) s = dag ? cfmac(s, u[3 * j + i], p[3 * a + j]) : cfma(s, u[3 * i + j], p[3 * a + j]);
-      o[3 * a + i] = s;
-    }
-}
-
-__device__ __forceinline__ double2 dot12(const double2 a[12], const double2 b[12]) {
-  double2 s = make_double2(0, 0);
-#pragma unroll
-  for (int c = 0; c < 12; c++) s = cfmac(s, a[c], b[c]);
-  return s;
-}
-
-struct ConnParams {
-  uint32_t V, sX, sT;
-  int T, LX, LY, LZ;
-  double ph_re[4], ph_im[4];
-};
 
 // one thread per (x, mu, k); blocks of 240 = 5 sites x 4 mu x 12 k
 __global__ void __launch_bounds__(240) k_conn(double2 *__restrict__ conn, const double2 *__restrict__ up,
@@ -224,29 +170,27 @@ __global__ void k_src_links(SrcLinks *out, const double2 *__restrict__ U, uint32
   out->u[nu][q] = cmul(U[(size_t)(nu * 9 + q) * V + y], make_double2(P.ph_re[nu], P.ph_im[nu]));
 }
 
-static ConnParams conn_params(const cvchip_ctx *c) {
-  ConnParams P;
-  P.V = (uint32_t)c->lat.V;
-  P.T = c->lat.T;
-  P.LX = c->lat.LX;
-  P.LY = c->lat.LY;
-  P.LZ = c->lat.LZ;
-  P.sX = (uint32_t)(c->lat.LY * c->lat.LZ);
-  P.sT = (uint32_t)(c->lat.LX * c->lat.LY * c->lat.LZ);
-  for (int m = 0; m < 4; m++) {
-    P.ph_re[m] = c->ph.re[m];
-    P.ph_im[m] = c->ph.im[m];
-  }
-  return P;
+// Source links Us_nu = U_nu(y) * phase_nu on the host (for kernels taking them by value).
+int source_links(cvchip_ctx *c, const int src[4], SrcLinks *out) {
+  SrcLinks *dL = nullptr;
+  CVC_HIP(hipMalloc(&dL, sizeof(SrcLinks)));
+  const uint32_t y = site_of(c, src[0], src[1], src[2], src[3]);
+  hipLaunchKernelGGL(k_src_links, dim3(1), dim3(64), 0, c->stream, dL, (const double2 *)c->gauge64,
+                     (uint32_t)c->lat.V, y, conn_params(c));
+  hipError_t e = hipMemcpyAsync(out, dL, sizeof(SrcLinks), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dL);
+  if (e != hipSuccess) return hip_fail(e, "source links", __FILE__, __LINE__);
+  return 0;
 }
 
-static uint32_t site_of(const cvchip_ctx *c, int t, int x, int y, int z) {
-  const Lattice &L = c->lat;
-  t = ((t % L.T) + L.T) % L.T;
-  x = ((x % L.LX) + L.LX) % L.LX;
-  y = ((y % L.LY) + L.LY) % L.LY;
-  z = ((z % L.LZ) + L.LZ) % L.LZ;
-  return (uint32_t)(((int64_t)(t * L.LX + x) * L.LY + y) * L.LZ + z);
+// Psi1 / Psi2 of one nu (24-column set psi) from the down propagators
+int launch_psi(cvchip_ctx *c, double2 *psi, const Field &dn, const SrcLinks &L, int nu) {
+  const uint32_t V = (uint32_t)c->lat.V;
+  hipLaunchKernelGGL(k_psi, dim3((V * 12u + 255) / 256), dim3(256), 0, c->stream, psi, (const double2 *)dn.data, L,
+                     nu, V);
+  CVC_HIP(hipGetLastError());
+  return 0;
 }
 
 // conn (host [V][16][2]) and contact from device up / dn sets of 60 columns
@@ -544,22 +488,17 @@ extern "C" int cvchip_pi_q(cvchip_ctx *c, double *piq, const double *conn, const
   return rc;
 }
 
-// The whole path for one source point: 5 x 12 up and 5 x 12 down solves with
-// the gamma5-hermitian BiCGStab (invert_Qtm_her), then the contraction.
-extern "C" int cvchip_cvc_source(cvchip_ctx *c, double *conn, double contact[8], const int src[4], double sp,
-                                 int niter_max, int prec, double delta, cvchip_stats *stats) {
-  if (!c || !conn || !contact || !src) {
-    set_error("NULL argument to cvchip_cvc_source");
-    return CVCHIP_ERR_NULL;
-  }
+namespace cvchip {
+
+// 5 x 12 up (mu) and 5 x 12 down (-mu) propagators from point sources at y + s^
+// (s = 0..3) and y (s = 4) into the 60-column sets up / dn (column 12 s + k),
+// with the gamma5-hermitian BiCGStab; statistics summed into tot.
+int solve_source_props(cvchip_ctx *c, const int src[4], double sp, int niter_max, int prec, double delta,
+                       Field &up, Field &dn, cvchip_stats *tot) {
   const uint32_t V = (uint32_t)c->lat.V;
-  Field up, dn, b, x;
-  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, b, 12, CVCHIP_PREC_F64);
+  Field b, x;
+  int rc = field_alloc(c, b, 12, CVCHIP_PREC_F64);
   if (!rc) rc = field_alloc(c, x, 12, CVCHIP_PREC_F64);
-  cvchip_stats tot;
-  memset(&tot, 0, sizeof(tot));
   const double mu0 = c->mu;
   for (int s = 0; s < 5 && !rc; s++) {
     const int d[4] = {s == 0, s == 1, s == 2, s == 3};
@@ -575,23 +514,44 @@ extern "C" int cvchip_cvc_source(cvchip_ctx *c, double *conn, double contact[8],
       c->mu = mu0;
       if (rc > 0) rc = 0;
       if (rc) break;
-      tot.iterations += st.iterations;
-      tot.converged += st.converged;
-      tot.reliable_updates += st.reliable_updates;
-      tot.dslash_calls += st.dslash_calls;
-      tot.seconds += st.seconds;
-      if (st.true_res2_max > tot.true_res2_max) tot.true_res2_max = st.true_res2_max;
+      if (tot) {
+        tot->iterations += st.iterations;
+        tot->converged += st.converged;
+        tot->reliable_updates += st.reliable_updates;
+        tot->dslash_calls += st.dslash_calls;
+        tot->seconds += st.seconds;
+        if (st.true_res2_max > tot->true_res2_max) tot->true_res2_max = st.true_res2_max;
+      }
       Field &dst = fl == 0 ? up : dn;
       hipLaunchKernelGGL(k_put_columns, dim3((144u * V + 255) / 256), dim3(256), 0, c->stream, (double2 *)dst.data,
                          (const double2 *)x.data, V, 60, 12 * s);
       if (hipGetLastError() != hipSuccess) rc = CVCHIP_ERR_HIP;
     }
   }
+  field_free(b);
+  field_free(x);
+  return rc;
+}
+
+}  // namespace cvchip
+
+// The whole path for one source point: 60 up and 60 down solves, then the
+// contraction.
+extern "C" int cvchip_cvc_source(cvchip_ctx *c, double *conn, double contact[8], const int src[4], double sp,
+                                 int niter_max, int prec, double delta, cvchip_stats *stats) {
+  if (!c || !conn || !contact || !src) {
+    set_error("NULL argument to cvchip_cvc_source");
+    return CVCHIP_ERR_NULL;
+  }
+  Field up, dn;
+  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+  cvchip_stats tot;
+  memset(&tot, 0, sizeof(tot));
+  if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
   if (!rc) rc = contract_fields(c, conn, contact, up, dn, src);
   if (stats) *stats = tot;
   field_free(up);
   field_free(dn);
-  field_free(b);
-  field_free(x);
   return rc;
 }

@@ -367,16 +367,18 @@ int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_o
 __global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int Q, int N,
                                   double *__restrict__ res) {
   __shared__ double sh[256];
-  const int q = blockIdx.x / N, r = blockIdx.x % N;
+  (void)Q;
+  (void)N;
+  const double *row = part + (size_t)blockIdx.x * nblk;   // (q, rhs) = (blockIdx / N, blockIdx % N)
   double a = 0.;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[((size_t)b * Q + q) * N + r];
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += row[b];
   sh[threadIdx.x] = a;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) res[q * N + r] = sh[0];
+  if (threadIdx.x == 0) res[blockIdx.x] = sh[0];
 }
 
 int reduce_partials(cvchip_ctx *c, const double *part, int nblk, int Q, int N, double *res) {

@@ -312,7 +312,9 @@ __global__ void k_sc_beta(RhsState *st, const double *res, int N, int it, int la
     // reliable update when the iterated residual fell by delta since the last
     // update, reached the tolerance (then decided in FP64), or has not done so
     // for kMaxGap iterations (a low-precision recurrence can stall above delta)
-    S.upd = (S.norm <= delta2 * S.rmax || S.norm <= S.tol2 || it - S.last_upd >= kMaxGap) ? 1 : 0;
+    // (the tolerance trigger waits for a factor 10 margin in |r|: the iterated
+    // low-precision residual runs slightly ahead of the true one)
+    S.upd = (S.norm <= delta2 * S.rmax || S.norm <= 0.01 * S.tol2 || it - S.last_upd >= kMaxGap) ? 1 : 0;
   }
   double w2[2], w3[2];
   cdiv(w2, w, S.r0rn);
@@ -355,6 +357,19 @@ __global__ void k_sc_r0rn(RhsState *st, const double *res, int N) {
   S.beta[1] = w2[0] * w3[1] + w2[1] * w3[0];
   S.r0rn[0] = w[0];
   S.r0rn[1] = w[1];
+}
+
+// mixed with restart (FP16 storage): after a reliable update the recurrence
+// starts afresh from the replaced residual: r2 = p = r, r0rn = |r|^2
+__global__ void k_sc_restart(RhsState *st, int N) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (S.upd && !S.done) {
+    S.r0rn[0] = S.norm;
+    S.r0rn[1] = 0.;
+  }
+  S.upd = 0;
 }
 
 __global__ void k_sc_clear_upd(RhsState *st, int N) {
@@ -615,6 +630,9 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
     solve_free(S);
   };
   Op opl{c, her, &auxl}, op64{c, her, &aux64};
+  // FP16 storage keeps too few digits for the Krylov space to survive a
+  // residual replacement: restart the recurrence at every reliable update
+  const bool restart = prec == CVCHIP_PREC_F16;
   if (rc) {
     cleanup();
     return rc;
@@ -679,16 +697,20 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
         TRY(op64.apply(t64, x, DOT_NONE, nullptr, nullptr));
         dcalls++;
         hipLaunchKernelGGL((k_residual<I64, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib,
-                           it64, 0, S.st, 1, S.VN, N, (double *)c->red.ptr);
+                           it64, restart ? 1 : 0, S.st, 1, S.VN, N, (double *)c->red.ptr);
         TRY(reduce(S, (int)S.nblk, 2));
         hipLaunchKernelGGL(k_sc_reliable, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it);
-        // r0rn and beta with the replaced residual: <r2, r>
-        hipLaunchKernelGGL((k_dot3<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir2, ir1, S.VN, N,
-                           (double *)c->red.ptr);
-        TRY(reduce(S, (int)S.nblk, 3));
-        hipLaunchKernelGGL(k_sc_r0rn, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
-        hipLaunchKernelGGL((k_p_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ip, ir1, ip2, S.st, S.VN, N);
-        hipLaunchKernelGGL(k_sc_clear_upd, dim3(1), dim3(256), 0, c->stream, S.st, N);
+        if (restart) {
+          hipLaunchKernelGGL(k_sc_restart, dim3(1), dim3(256), 0, c->stream, S.st, N);
+        } else {
+          // r0rn and beta with the replaced residual: <r2, r>
+          hipLaunchKernelGGL((k_dot3<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir2, ir1, S.VN, N,
+                             (double *)c->red.ptr);
+          TRY(reduce(S, (int)S.nblk, 3));
+          hipLaunchKernelGGL(k_sc_r0rn, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+          hipLaunchKernelGGL((k_p_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ip, ir1, ip2, S.st, S.VN, N);
+          hipLaunchKernelGGL(k_sc_clear_upd, dim3(1), dim3(256), 0, c->stream, S.st, N);
+        }
         CVC_HIP(hipGetLastError());
         TRY(pull_state(S));
       }

@@ -108,12 +108,11 @@ struct IOHalf {
 
 // Sums Q per-thread values over the threads of a block that share a
 // right-hand side (thread t holds rhs t % N; blockDim.x is a multiple of N)
-// with a fixed-order tree, and writes part[(slot*Q + q)*N + rhs] (slot defaults
-// to the block index).
+// with a fixed-order tree, and writes part[(q*N + rhs)*gridDim.x + block], so
+// that the second stage reads each (q, rhs) row contiguously.
 template <int Q>
-__device__ __forceinline__ void block_reduce_store(const double v[Q], double *__restrict__ part, int N,
-                                                   uint32_t slot = 0xffffffffu) {
-  if (slot == 0xffffffffu) slot = blockIdx.x;
+__device__ __forceinline__ void block_reduce_store(const double v[Q], double *__restrict__ part, int N) {
+  const uint32_t slot = blockIdx.x, nb = gridDim.x;
   __shared__ double sh[Q][256];
   const int tid = threadIdx.x;
   const int B = blockDim.x;
@@ -133,11 +132,11 @@ __device__ __forceinline__ void block_reduce_store(const double v[Q], double *__
   }
   if (tid < N) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) part[((size_t)slot * Q + q) * N + tid] = sh[q][tid];
+    for (int q = 0; q < Q; q++) part[((size_t)q * N + tid) * nb + slot] = sh[q][tid];
   }
 }
 
-// Reduces part[nblk][Q][N] over blocks into res[q*N + rhs]; launched with one
+// Reduces part[Q][N][nblk] over blocks into res[q*N + rhs]; launched with one
 // block of 256 threads per (q, rhs); fixed order, so results are reproducible.
 __global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int Q, int N,
                                   double *__restrict__ res);

@@ -174,6 +174,19 @@ int cvchip_cvc_source(cvchip_ctx *ctx, double *conn, double contact[8], const in
                       double sp, int niter_max, int prec, double delta,
                       cvchip_stats *stats);
 
+/* Gradient of L = sum_mu sum_{q in Q} Re Pi_{mu mu}(q) (Pi(q) as in cvchip_pi_q,
+ * Q = {q : |q| <= qmax}, q_mu = 2 pi n_mu / L_mu with n_mu in (-L_mu/2, L_mu/2])
+ * with respect to the twisted mass mu and every gauge link, by adjoint solves:
+ * 120 forward and 120 adjoint gamma5-hermitian BiCGStab solves (prec / delta
+ * as for cvchip_invert_Qtm), adjoint sources from the contraction, and
+ * link-derivative outer products.  No reference counterpart.
+ * Outputs: *L, *dL_dmu, and (if force != NULL) force[V][4][18]: per link the
+ * traceless Hermitian 3x3 matrix F (row-major complex) with
+ * dL/d eps = tr(T F) for U_mu(x) -> exp(i eps T) U_mu(x), T traceless Hermitian. */
+int cvchip_cvc_gradient(cvchip_ctx *ctx, const int src[4], double qmax, double sp, int niter_max,
+                        int prec, double delta, double *L, double *dL_dmu, double *force,
+                        cvchip_stats *stats);
+
 /* -------------------------------------------------------------- inputs */
 
 /* Seeded gauge field U_mu(x) = exp(i eps H) in SU(3), H traceless Hermitian
