@@ -22,6 +22,7 @@ ERR_NULL, ERR_NOCONV = -2, -3
 # every entry point declared in include/cvchip.h
 EXPORTS = [
     "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
+    "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
     "cvchip_field_destroy", "cvchip_field_upload", "cvchip_field_download", "cvchip_field_copy",
@@ -64,6 +65,7 @@ def lib():
         L.cvchip_ctx_destroy.argtypes = [_vp]
         L.cvchip_ctx_set_params.argtypes = [_vp, C.c_double, C.c_double]
         L.cvchip_ctx_set_link_recon.argtypes = [_vp, C.c_int]
+        L.cvchip_ctx_set_dslash_policy.argtypes = [_vp, C.c_int]
         L.cvchip_set_gauge.argtypes = [_vp, _dp]
         L.cvchip_Q_phi.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int]
         L.cvchip_scalar_product_co.argtypes = [_vp, _dp, _dp, _dp]
@@ -198,6 +200,9 @@ class Context:
     def set_link_recon(self, recon: int):
         _check(lib().cvchip_ctx_set_link_recon(self._h, int(recon)))
 
+    def set_dslash_policy(self, policy: int):
+        _check(lib().cvchip_ctx_set_dslash_policy(self._h, int(policy)))
+
     def set_gauge(self, gauge: np.ndarray):
         g = np.ascontiguousarray(gauge, dtype=np.float64)
         assert g.size == self.V * 72
@@ -271,10 +276,11 @@ class Context:
             raise CvchipError(rc, last_error())
         return rc, st.as_dict()
 
-    def invert_her(self, x: Field, b: Field, sp, niter_max=1000, prec=PREC_F64, delta=0.1, check=True):
+    def invert_her(self, x: Field, b: Field, sp, niter_max=1000, prec=PREC_F64, delta=0.1, check=True,
+                   verbose=False):
         st = Stats()
         rc = lib().cvchip_field_invert_her(self._h, x._h, b._h, float(sp), int(niter_max), int(prec),
-                                           float(delta), 0, C.byref(st))
+                                           float(delta), SOLVE_VERBOSE if verbose else 0, C.byref(st))
         if check and rc < 0:
             raise CvchipError(rc, last_error())
         return rc, st.as_dict()

@@ -248,6 +248,16 @@ extern "C" int cvchip_ctx_set_link_recon(cvchip_ctx *c, int recon) {
   return CVCHIP_OK;
 }
 
+extern "C" int cvchip_ctx_set_dslash_policy(cvchip_ctx *c, int policy) {
+  if (!c) return CVCHIP_ERR_NULL;
+  if (policy < 0 || policy > 3) {
+    set_error("operator staging policy must be 0..3, got %d", policy);
+    return CVCHIP_ERR_ARG;
+  }
+  c->dslash_policy = policy;
+  return CVCHIP_OK;
+}
+
 extern "C" int cvchip_set_gauge(cvchip_ctx *c, const double *gauge) {
   if (!c || !gauge) {
     set_error("NULL argument to cvchip_set_gauge");

@@ -99,24 +99,48 @@ __device__ __forceinline__ void hop(cplx<R> acc[12], const cplx<R> psi[12], cons
 // U_mu(x - mu) (backward), 2*mu + 1 is U_mu(x) (forward).
 constexpr int LDS_SITES = 32;
 
-template <typename IO, int RECON, int MU, bool LDSL>
-__device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const IO &in,
+// Neighbour spinor loads.  With LDSS the block's own sites and the two sites
+// before and after them are staged in LDS, so the z neighbours (adjacent
+// sites) and the own spinor come from LDS instead of the texture path.
+template <typename IO, bool LDSS>
+struct NbLoader {
+  using R = typename IO::real;
+  const IO &in;
+  const cplx<R> *Sp;
+  uint32_t N, r, s0, S;
+  __device__ __forceinline__ void operator()(cplx<R> v[12], uint32_t site, bool from_lds) const {
+    if constexpr (LDSS) {
+      if (from_lds) {
+        const int64_t d = (int64_t)site - (int64_t)s0 + 1;
+        if (d >= 0 && d <= (int64_t)S + 1) {
+#pragma unroll
+          for (int c = 0; c < 12; c++) v[c] = Sp[((uint32_t)d * 12 + c) * N + r];
+          return;
+        }
+      }
+    }
+    in.load(v, site * N + r);
+  }
+};
+
+template <typename IO, int RECON, int MU, bool LDSL, bool LDSS>
+__device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const NbLoader<IO, LDSS> &ld,
                                          const cplx<typename IO::real> *__restrict__ U,
                                          const cplx<typename IO::real> *__restrict__ Ul,
                                          const DParams<typename IO::real> &P, uint32_t s, uint32_t sl,
-                                         uint32_t r, int coord, int L, uint32_t stride) {
+                                         int coord, int L, uint32_t stride) {
   using R = typename IO::real;
   constexpr int KL = RECON == 18 ? 9 : 6;
   cplx<R> psi[12], u[9];
   const uint32_t sm = coord == 0 ? s + (uint32_t)(L - 1) * stride : s - stride;
   const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
   // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
-  in.load(psi, sm * P.N + r);
+  ld(psi, sm, MU == 3);
   if constexpr (LDSL) load_link<R, RECON>(u, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES);
   else load_link<R, RECON>(u, U + (size_t)MU * KL * P.V + sm, P.V);
   hop<R, MU, false>(acc, psi, u, P.ph[MU]);
   // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
-  in.load(psi, sp * P.N + r);
+  ld(psi, sp, MU == 3);
   if constexpr (LDSL) load_link<R, RECON>(u, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES);
   else load_link<R, RECON>(u, U + (size_t)MU * KL * P.V + s, P.V);
   hop<R, MU, true>(acc, psi, u, P.ph[MU]);
@@ -137,7 +161,7 @@ template <typename R> struct DslashWaves {
 // LDSL: the block's sites share their links through LDS (N >= 8, at most
 // LDS_SITES sites per block): every link entry crosses the texture path once
 // per block instead of once per rhs lane.
-template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL>
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL, bool LDSS>
 __global__ void __launch_bounds__(256, DslashWaves<typename IOI::real>::value)
 k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParams<typename IOI::real> P,
          IOI r2, double *__restrict__ part) {
@@ -180,6 +204,22 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     }
     __syncthreads();
   }
+  extern __shared__ unsigned char dyn_lds[];
+  C *Sp = reinterpret_cast<C *>(dyn_lds);
+  if constexpr (LDSS) {
+    // stage sites s0-1 .. s0+S (all rhs); (s0 - 1 + k) mod V, every index is a valid site
+    const uint32_t nst = (S + 2) * (uint32_t)P.N;
+    for (uint32_t u = threadIdx.x; u < nst; u += blockDim.x) {
+      const uint32_t k = u / (uint32_t)P.N, rr = u - k * (uint32_t)P.N;
+      const uint32_t site = (s0 + P.V - 1 + k) % P.V;
+      C v[12];
+      in.load(v, site * (uint32_t)P.N + rr);
+#pragma unroll
+      for (int c = 0; c < 12; c++) Sp[(k * 12 + c) * (uint32_t)P.N + rr] = v[c];
+    }
+    __syncthreads();
+  }
+  const NbLoader<IOI, LDSS> ld{in, Sp, (uint32_t)P.N, r, s0, S};
   if (s < s_end) {
     const uint32_t t = s / P.sT;
     uint32_t rem = s - t * P.sT;
@@ -192,13 +232,13 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
 #pragma unroll
     for (int c = 0; c < 12; c++) acc[c] = mk<R>(0, 0);
     const uint32_t sl = s - s0;
-    hop_pair<IOI, RECON, 0, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)t, P.T, P.sT);
-    hop_pair<IOI, RECON, 1, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)x, P.LX, P.sX);
-    hop_pair<IOI, RECON, 2, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)y, P.LY, (uint32_t)P.LZ);
-    hop_pair<IOI, RECON, 3, LDSL>(acc, in, U, Ul, P, s, sl, r, (int)z, P.LZ, 1u);
+    hop_pair<IOI, RECON, 0, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)t, P.T, P.sT);
+    hop_pair<IOI, RECON, 1, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)x, P.LX, P.sX);
+    hop_pair<IOI, RECON, 2, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)y, P.LY, (uint32_t)P.LZ);
+    hop_pair<IOI, RECON, 3, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)z, P.LZ, 1u);
 
     C own[12];
-    in.load(own, i);
+    ld(own, s, true);
     const R mh = (R)-0.5;
 #pragma unroll
     for (int c = 0; c < 12; c++) {
@@ -314,11 +354,16 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   IOI ii = make_io<IOI>(in);
   IOI rr = r2 ? make_io<IOI>(*r2) : ii;
   dim3 g(nblk), b(B);
-  const bool ldsl = sizeof(R) == 4 && B / N <= LDS_SITES;
-#define CVC_LAUNCH(G5, DMV)                                                                               \
-  do {                                                                                                    \
-    if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
-    else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
+  const bool multi = B / N <= LDS_SITES;
+  const bool ldsl = multi && (c->dslash_policy & 1);
+  const bool ldss = multi && (c->dslash_policy & 2);
+  const size_t dyn = ldss ? (size_t)(B / N + 2) * N * 12 * sizeof(cplx<R>) : 0;
+#define CVC_LAUNCH(G5, DMV)                                                                                     \
+  do {                                                                                                          \
+    if (ldsl && ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, true>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
+    else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
+    else if (ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, true>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
+    else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
   } while (0)
   if (g5) {
     if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);

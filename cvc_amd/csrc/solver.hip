@@ -372,6 +372,13 @@ __global__ void k_sc_restart(RhsState *st, int N) {
   S.upd = 0;
 }
 
+// the FP64 residual recompute runs over all rhs at once: give every active rhs
+// its reliable update in the same pass
+__global__ void k_sc_upd_all(RhsState *st, int N) {
+  const int r = threadIdx.x;
+  if (r < N && !st[r].done) st[r].upd = 1;
+}
+
 __global__ void k_sc_clear_upd(RhsState *st, int N) {
   const int r = threadIdx.x;
   if (r < N) st[r].upd = 0;
@@ -610,7 +617,7 @@ static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
 // gamma5-hermitian system when her != 0, b then being gamma5 b).
 template <typename IOL>
 static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec,
-                          double delta, cvchip_stats *stats, int her = 0) {
+                          double delta, cvchip_stats *stats, int her = 0, int verbose = 0) {
   const int N = b.nrhs;
   Solve S;
   int rc = solve_setup(c, S, N);
@@ -690,8 +697,9 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
       int any_upd = 0;
       for (int r = 0; r < N; r++) any_upd |= (!S.hst[r].done && S.hst[r].upd);
       if (any_upd) {
-        // reliable update for flagged rhs: x += xl; xl = 0; r = b - D64 x
+        // reliable update for all active rhs: x += xl; xl = 0; r = b - D64 x
         nrel++;
+        hipLaunchKernelGGL(k_sc_upd_all, dim3(1), dim3(256), 0, c->stream, S.st, N);
         hipLaunchKernelGGL((k_accum_zero<I64, IOPlain<float>>), dim3(S.nblk), dim3(S.B), 0, c->stream, ix, ixl, S.st,
                            S.VN, N);
         TRY(op64.apply(t64, x, DOT_NONE, nullptr, nullptr));
@@ -700,6 +708,14 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
                            it64, restart ? 1 : 0, S.st, 1, S.VN, N, (double *)c->red.ptr);
         TRY(reduce(S, (int)S.nblk, 2));
         hipLaunchKernelGGL(k_sc_reliable, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it);
+        if (verbose) {
+          TRY(pull_state(S));
+          double mx = 0.;
+          for (int r = 0; r < N; r++)
+            if (S.hst[r].normb > 0 && S.hst[r].norm / S.hst[r].normb > mx) mx = S.hst[r].norm / S.hst[r].normb;
+          printf("# [mixed] reliable update %d at iteration %d: max true |r|^2/|b|^2 = %e\n", nrel, it, mx);
+          fflush(stdout);
+        }
         if (restart) {
           hipLaunchKernelGGL(k_sc_restart, dim3(1), dim3(256), 0, c->stream, S.st, N);
         } else {
@@ -774,9 +790,11 @@ int invert_her_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
         set_error("reliable-update factor delta must lie in (0,1), got %g", delta);
         rc = CVCHIP_ERR_ARG;
       } else if (prec == CVCHIP_PREC_F32)
-        rc = bicgstab_mixed<IOPlain<float>>(c, y, g5b, sp, niter_max, prec, delta, stats, 1);
+        rc = bicgstab_mixed<IOPlain<float>>(c, y, g5b, sp, niter_max, prec, delta, stats, 1,
+                                            (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0);
       else if (prec == CVCHIP_PREC_F16)
-        rc = bicgstab_mixed<IOHalf>(c, y, g5b, sp, niter_max, prec, delta, stats, 1);
+        rc = bicgstab_mixed<IOHalf>(c, y, g5b, sp, niter_max, prec, delta, stats, 1,
+                                    (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0);
       else {
         set_error("unknown precision %d", prec);
         rc = CVCHIP_ERR_ARG;

@@ -66,6 +66,9 @@ int cvchip_ctx_destroy(cvchip_ctx *ctx);
 int cvchip_ctx_set_params(cvchip_ctx *ctx, double kappa, double mu);
 /* link storage of the FP64 operator: 18 (full) or 12 (two rows, third rebuilt) */
 int cvchip_ctx_set_link_recon(cvchip_ctx *ctx, int recon);
+/* staging of the multi-rhs operator (N >= 8): bit 0 keeps a block's links in
+ * LDS, bit 1 keeps its own and z-neighbour spinors in LDS; default 3 */
+int cvchip_ctx_set_dslash_policy(cvchip_ctx *ctx, int policy);
 const char *cvchip_last_error(void);
 
 /* Upload g_gauge_field (read_lime_gauge_field_doubleprec, io.c:202-347, fills it
