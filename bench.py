@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--no-solve", action="store_true", help="operator timing only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the mixed solver")
+    ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
+    ap.add_argument("--delta16", type=float, default=0.1, help="reliable-update factor of the FP16 mixed solver")
     return ap.parse_args()
 
 
@@ -115,7 +116,7 @@ def main():
             torch.cuda.synchronize()
             s0 = time.perf_counter()
             rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
-                                    delta=a.delta, check=False)
+                                    delta=a.delta16 if prec == cvc_amd.PREC_F16 else a.delta, check=False)
             torch.cuda.synchronize()
             el = time.perf_counter() - s0
             extra[f"solve_{name}_s"] = round(el, 3)

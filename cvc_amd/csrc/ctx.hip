@@ -250,8 +250,13 @@ extern "C" int cvchip_ctx_set_link_recon(cvchip_ctx *c, int recon) {
 
 extern "C" int cvchip_ctx_set_dslash_policy(cvchip_ctx *c, int policy) {
   if (!c) return CVCHIP_ERR_NULL;
-  if (policy < 0 || policy > 3) {
-    set_error("operator staging policy must be 0..3, got %d", policy);
+  const int mode = (policy >> 4) & 15, yb = (policy >> 8) & 255;
+  if (policy < 0 || policy > 0xffff || (policy & 0xc) || mode > 3) {
+    set_error("operator policy 0x%x: bits 0-1 staging, 4-7 traversal mode 0..3, 8-15 y-block", policy);
+    return CVCHIP_ERR_ARG;
+  }
+  if (yb && c->lat.LY % yb) {
+    set_error("operator policy: y-block %d does not divide LY = %d", yb, c->lat.LY);
     return CVCHIP_ERR_ARG;
   }
   c->dslash_policy = policy;

@@ -21,77 +21,87 @@ struct DParams {
   uint32_t V, VN, sX, sT;
   int N, T, LX, LY, LZ;
   int partner;   // DOT_TS: dot with the separate field r2 instead of the input
-  // traversal: the lattice is cut into chunks of csz consecutive sites (one
-  // eighth of a time slice when possible), bpc blocks per chunk; with xcd set
+  // traversal (mode 1): the lattice is cut into chunks of csz consecutive
+  // sites (one eighth of a time slice when possible), bpc blocks per chunk;
   // the 8 XCDs take the 8 chunks of a time slice side by side, so the whole
-  // GPU advances through the slices together (t neighbours stay in the MALL)
-  // while each XCD's L2 holds the x/y/z neighbours of its own chunk
+  // GPU advances through the slices together.  Mode 0: the same chunks with
+  // no XCD interleave.  Mode 2: XCD j owns the time slices t = j mod 8 and
+  // walks each slice in runs of yb consecutive y rows (yb*LZ consecutive
+  // sites), x within y block: the x and y neighbours of a run were touched a
+  // few runs earlier and are still in the XCD's L2, and the t neighbours are
+  // the slices the adjacent XCDs are working on at the same time.
   uint32_t csz, bpc;
-  int xcd;
+  int mode;
+  uint32_t yb, runlen, bpr, nrun;
   R k2, mutm;
   cplx<R> ph[4];
 };
 
-// u[k] = base[k * stride] for the stored entries (9, or the first two rows)
-template <typename R, int RECON>
-__device__ __forceinline__ void load_link(cplx<R> u[9], const cplx<R> *__restrict__ base, uint32_t stride) {
-  if constexpr (RECON == 18) {
+// Spin projection of a neighbour spinor: h[a] = psi_a -+ g_a psi_{GCOL[a]}
+// for the upper spins a = 0, 1 ((1 -+ gamma_mu) has rank 2 in the chiral
+// basis).  get(c) returns component c = 3*spin + colour; FP64 loads each
+// component where it is used, so no more than the half spinor and one pair of
+// spin rows are live at a time.
+template <int MU, bool FWD, typename C, typename Get>
+__device__ __forceinline__ void project(C h[2][3], Get get) {
+  constexpr int p0 = GCOL[MU][0], p1 = GCOL[MU][1];
+  constexpr int g0 = FWD ? uneg(GVAL[MU][0]) : GVAL[MU][0];
+  constexpr int g1 = FWD ? uneg(GVAL[MU][1]) : GVAL[MU][1];
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = base[(size_t)k * stride];
-  } else {
+  for (int c = 0; c < 3; c++) h[0][c] = cadd(get(c), cunit<g0>(get(3 * p0 + c)));
+#pragma unroll
+  for (int c = 0; c < 3; c++) h[1][c] = cadd(get(3 + c), cunit<g1>(get(3 * p1 + c)));
+}
+
+// acc += (1 - gamma_MU) U ph psi (FWD) or (1 + gamma_MU) U^dagger conj(ph) psi
+// (!FWD), from the projected h: one colour row of the link at a time,
+// w_a = row . h_a, then the upper spins take w_a and the lower spins
+// -+g_b w_{GCOL[b]}.  base[k * stride] is the stored link entry k (9 entries,
+// or the first two rows, the third rebuilt as conj(row0 x row1)).
+template <typename R, int RECON, int MU, bool FWD>
+__device__ __forceinline__ void link_apply(cplx<R> acc[12], const cplx<R> h[2][3],
+                                           const cplx<R> *__restrict__ base, uint32_t stride, cplx<R> ph) {
+  using C = cplx<R>;
+  constexpr int q2 = GCOL[MU][2], q3 = GCOL[MU][3];
+  constexpr int l2 = FWD ? uneg(GVAL[MU][2]) : GVAL[MU][2];
+  constexpr int l3 = FWD ? uneg(GVAL[MU][3]) : GVAL[MU][3];
+  C u[9];
+  if constexpr (RECON == 12) {
 #pragma unroll
     for (int k = 0; k < 6; k++) u[k] = base[(size_t)k * stride];
-    // SU(3): row2 = conj(row0 x row1)
-    cplx<R> a = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
-    cplx<R> b = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
-    cplx<R> d = csub(cmul(u[0], u[4]), cmul(u[1], u[3]));
+    const C a = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
+    const C b = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
+    const C d = csub(cmul(u[0], u[4]), cmul(u[1], u[3]));
     u[6] = mk<R>(a.x, -a.y);
     u[7] = mk<R>(b.x, -b.y);
     u[8] = mk<R>(d.x, -d.y);
   }
-}
-
-// acc += (1 - gamma_MU) U ph psi   (FWD)   or   (1 + gamma_MU) U^dagger conj(ph) psi   (!FWD)
-template <typename R, int MU, bool FWD>
-__device__ __forceinline__ void hop(cplx<R> acc[12], const cplx<R> psi[12], const cplx<R> u[9],
-                                    cplx<R> ph) {
-  using C = cplx<R>;
-  constexpr int p0 = GCOL[MU][0], p1 = GCOL[MU][1];
-  constexpr int g0 = FWD ? uneg(GVAL[MU][0]) : GVAL[MU][0];
-  constexpr int g1 = FWD ? uneg(GVAL[MU][1]) : GVAL[MU][1];
-  C h[2][3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    h[0][c] = cadd(psi[c], cunit<g0>(psi[3 * p0 + c]));
-    h[1][c] = cadd(psi[3 + c], cunit<g1>(psi[3 * p1 + c]));
-  }
-  C v[2][3];
-#pragma unroll
-  for (int a = 0; a < 2; a++) {
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      C s = mk<R>(0, 0);
-      if constexpr (FWD) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) s = cfma(s, u[3 * i + j], h[a][j]);
-        v[a][i] = cmul(ph, s);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 3; j++) s = cfmac(s, u[3 * j + i], h[a][j]);
-        v[a][i] = cmulc(ph, s);
-      }
-    }
-  }
-  // lower components: (1 -+ gamma) psi at row b equals -+g_b h[p(b)]
-  constexpr int q2 = GCOL[MU][2], q3 = GCOL[MU][3];
-  constexpr int l2 = FWD ? uneg(GVAL[MU][2]) : GVAL[MU][2];
-  constexpr int l3 = FWD ? uneg(GVAL[MU][3]) : GVAL[MU][3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    acc[i] = cadd(acc[i], v[0][i]);
-    acc[3 + i] = cadd(acc[3 + i], v[1][i]);
-    acc[6 + i] = cadd(acc[6 + i], cunit<l2>(v[q2][i]));
-    acc[9 + i] = cadd(acc[9 + i], cunit<l3>(v[q3][i]));
+    C w0 = mk<R>(0, 0), w1 = mk<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int k = FWD ? 3 * i + j : 3 * j + i;
+      const C e = RECON == 18 ? base[(size_t)k * stride] : u[k];
+      if constexpr (FWD) {
+        w0 = cfma(w0, e, h[0][j]);
+        w1 = cfma(w1, e, h[1][j]);
+      } else {
+        w0 = cfmac(w0, e, h[0][j]);
+        w1 = cfmac(w1, e, h[1][j]);
+      }
+    }
+    if constexpr (FWD) {
+      w0 = cmul(ph, w0);
+      w1 = cmul(ph, w1);
+    } else {
+      w0 = cmulc(ph, w0);
+      w1 = cmulc(ph, w1);
+    }
+    acc[i] = cadd(acc[i], w0);
+    acc[3 + i] = cadd(acc[3 + i], w1);
+    acc[6 + i] = cadd(acc[6 + i], cunit<l2>(q2 == 0 ? w0 : w1));
+    acc[9 + i] = cadd(acc[9 + i], cunit<l3>(q3 == 0 ? w0 : w1));
   }
 }
 
@@ -99,29 +109,72 @@ __device__ __forceinline__ void hop(cplx<R> acc[12], const cplx<R> psi[12], cons
 // U_mu(x - mu) (backward), 2*mu + 1 is U_mu(x) (forward).
 constexpr int LDS_SITES = 32;
 
-// Neighbour spinor loads.  With LDSS the block's own sites and the two sites
-// before and after them are staged in LDS, so the z neighbours (adjacent
-// sites) and the own spinor come from LDS instead of the texture path.
+// Projected neighbour spinors.  With LDSS the block's own sites and the two
+// sites before and after them are staged in LDS, so the z neighbours
+// (adjacent sites) and the own spinor come from LDS instead of the texture
+// path.
 template <typename IO, bool LDSS>
 struct NbLoader {
   using R = typename IO::real;
+  using C = cplx<R>;
   const IO &in;
-  const cplx<R> *Sp;
+  const C *Sp;
   uint32_t N, r, s0, S;
-  __device__ __forceinline__ void operator()(cplx<R> v[12], uint32_t site, bool from_lds) const {
+  // the staged slot of site, or -1
+  __device__ __forceinline__ int slot(uint32_t site, bool from_lds) const {
     if constexpr (LDSS) {
       if (from_lds) {
         const int64_t d = (int64_t)site - (int64_t)s0 + 1;
-        if (d >= 0 && d <= (int64_t)S + 1) {
-#pragma unroll
-          for (int c = 0; c < 12; c++) v[c] = Sp[((uint32_t)d * 12 + c) * N + r];
-          return;
-        }
+        if (d >= 0 && d <= (int64_t)S + 1) return (int)d;
       }
+    }
+    return -1;
+  }
+  template <int MU, bool FWD>
+  __device__ __forceinline__ void proj(C h[2][3], uint32_t site, bool from_lds) const {
+    const int d = slot(site, from_lds);
+    if (d >= 0) {
+      const C *q = Sp + (uint32_t)d * 12 * N + r;
+      project<MU, FWD>(h, [&](int c) { return q[c * N]; });
+      return;
+    }
+    const uint32_t idx = site * N + r;
+    if constexpr (IO::K == 1) {
+      project<MU, FWD>(h, [&](int c) { return in.load_c(c, idx); });
+    } else {
+      C v[12];
+      in.load(v, idx);
+      project<MU, FWD>(h, [&](int c) { return v[c]; });
+    }
+  }
+  __device__ __forceinline__ void full(C v[12], uint32_t site, bool from_lds) const {
+    const int d = slot(site, from_lds);
+    if (d >= 0) {
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = Sp[((uint32_t)d * 12 + c) * N + r];
+      return;
     }
     in.load(v, site * N + r);
   }
 };
+
+// Multi-rhs operator (links from LDS): the compiler would otherwise issue the
+// loads of all eight hops up front and defer the arithmetic to the epilogue
+// (the accumulator is not needed before), keeping every loaded value live,
+// spilling, and running one wave per SIMD.  An empty asm that reads and writes
+// the accumulator (and, at PIN 2, the projected half spinor) and clobbers
+// memory bounds each hop: its arithmetic is done before, and the next hop's
+// loads cannot move above it, so three to four waves fit per SIMD and cover
+// each other's latency.  The single-rhs operator keeps the compiler's
+// schedule (all loads in flight, one wave per SIMD): it is faster there.
+#ifndef CVC_HOP_PIN
+#define CVC_HOP_PIN 2
+#endif
+template <typename C>
+__device__ __forceinline__ void pin_regs(C *v, int n) {
+#pragma unroll
+  for (int k = 0; k < n; k++) asm volatile("" : "+v"(v[k].x), "+v"(v[k].y) : : "memory");
+}
 
 template <typename IO, int RECON, int MU, bool LDSL, bool LDSS>
 __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const NbLoader<IO, LDSS> &ld,
@@ -131,19 +184,21 @@ __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const 
                                          int coord, int L, uint32_t stride) {
   using R = typename IO::real;
   constexpr int KL = RECON == 18 ? 9 : 6;
-  cplx<R> psi[12], u[9];
+  cplx<R> h[2][3];
   const uint32_t sm = coord == 0 ? s + (uint32_t)(L - 1) * stride : s - stride;
   const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
   // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
-  ld(psi, sm, MU == 3);
-  if constexpr (LDSL) load_link<R, RECON>(u, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES);
-  else load_link<R, RECON>(u, U + (size_t)MU * KL * P.V + sm, P.V);
-  hop<R, MU, false>(acc, psi, u, P.ph[MU]);
+  ld.template proj<MU, false>(h, sm, MU == 3);
+  if constexpr (LDSL && CVC_HOP_PIN > 1) pin_regs(&h[0][0], 6);
+  if constexpr (LDSL) link_apply<R, RECON, MU, false>(acc, h, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
+  else link_apply<R, RECON, MU, false>(acc, h, U + (size_t)MU * KL * P.V + sm, P.V, P.ph[MU]);
+  if constexpr (LDSL && CVC_HOP_PIN > 0) pin_regs(acc, 12);
   // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
-  ld(psi, sp, MU == 3);
-  if constexpr (LDSL) load_link<R, RECON>(u, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES);
-  else load_link<R, RECON>(u, U + (size_t)MU * KL * P.V + s, P.V);
-  hop<R, MU, true>(acc, psi, u, P.ph[MU]);
+  ld.template proj<MU, true>(h, sp, MU == 3);
+  if constexpr (LDSL && CVC_HOP_PIN > 1) pin_regs(&h[0][0], 6);
+  if constexpr (LDSL) link_apply<R, RECON, MU, true>(acc, h, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
+  else link_apply<R, RECON, MU, true>(acc, h, U + (size_t)MU * KL * P.V + s, P.V, P.ph[MU]);
+  if constexpr (LDSL && CVC_HOP_PIN > 0) pin_regs(acc, 12);
 }
 
 // Minimum resident waves per SIMD the register allocation must allow (the
@@ -169,17 +224,30 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
   using C = cplx<R>;
   constexpr int KL = RECON == 18 ? 9 : 6;
   const uint32_t S = blockDim.x / (uint32_t)P.N;
-  uint32_t chunk, blk;
-  if (P.xcd) {
-    const uint32_t j = blockIdx.x >> 3;
-    chunk = (j / P.bpc) * kXcds + (blockIdx.x & (kXcds - 1));
-    blk = j % P.bpc;
+  uint32_t s0, s_end;
+  if (P.mode == 2) {
+    const uint32_t k = blockIdx.x / kXcds;
+    const uint32_t per_slice = P.nrun * P.bpr;
+    const uint32_t tl = k / per_slice, rem = k - tl * per_slice;
+    const uint32_t run = rem / P.bpr, blk = rem - run * P.bpr;
+    const uint32_t t = tl * kXcds + blockIdx.x % kXcds;
+    const uint32_t ybl = run / (uint32_t)P.LX, x = run - ybl * (uint32_t)P.LX;
+    const uint32_t start = ((t * (uint32_t)P.LX + x) * (uint32_t)P.LY + ybl * P.yb) * (uint32_t)P.LZ;
+    s0 = start + blk * S;
+    s_end = min(s0 + S, start + P.runlen);
   } else {
-    chunk = blockIdx.x / P.bpc;
-    blk = blockIdx.x % P.bpc;
+    uint32_t chunk, blk;
+    if (P.mode == 1) {
+      const uint32_t j = blockIdx.x / kXcds;
+      chunk = (j / P.bpc) * kXcds + blockIdx.x % kXcds;
+      blk = j % P.bpc;
+    } else {
+      chunk = blockIdx.x / P.bpc;
+      blk = blockIdx.x % P.bpc;
+    }
+    s0 = chunk * P.csz + blk * S;
+    s_end = min(s0 + S, (chunk + 1) * P.csz);
   }
-  const uint32_t s0 = chunk * P.csz + blk * S;
-  const uint32_t s_end = min(s0 + S, (chunk + 1) * P.csz);
   const uint32_t r = threadIdx.x % (uint32_t)P.N;
   const uint32_t s = s0 + threadIdx.x / (uint32_t)P.N;
   const uint32_t i = s * (uint32_t)P.N + r;
@@ -238,7 +306,7 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     hop_pair<IOI, RECON, 3, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)z, P.LZ, 1u);
 
     C own[12];
-    ld(own, s, true);
+    ld.full(own, s, true);
     const R mh = (R)-0.5;
 #pragma unroll
     for (int c = 0; c < 12; c++) {
@@ -311,17 +379,40 @@ template <> IOHalf make_io<IOHalf>(const Field &f) {
 
 struct Traversal {
   uint32_t csz, bpc, nblk;
-  int xcd;
+  int mode;
+  uint32_t yb, runlen, bpr, nrun;
 };
 
+// Block-to-site mapping; the policy's bits 4-7 pick the mode (0: automatic)
+// and bits 8-15 the y-block height of mode 2 (0: automatic).
 static Traversal traversal(const cvchip_ctx *c, int N) {
   const Lattice &L = c->lat;
   const uint32_t V = (uint32_t)L.V, slice = (uint32_t)(L.LX * L.LY * L.LZ);
   const uint32_t S = (uint32_t)(block_for(N) / N);
-  Traversal t;
-  if (slice % kXcds == 0) { t.csz = slice / kXcds; t.xcd = 1; }
-  else if (V % kXcds == 0) { t.csz = V / kXcds; t.xcd = 1; }
-  else { t.csz = V; t.xcd = 0; }
+  int want = (c->dslash_policy >> 4) & 15;
+  uint32_t yb = (uint32_t)((c->dslash_policy >> 8) & 255);
+  // automatic: slice per XCD in 16-row blocks for the multi-rhs operator
+  // (measured best at 64x32^3 with 12 rhs), XCD chunks of a slice otherwise
+  if (want == 0) want = (N >= 8 && L.T % (int)kXcds == 0) ? 2 : 1;
+  if (yb == 0) {
+    yb = 1;
+    for (uint32_t y : {16u, 8u, 4u, 2u})
+      if (L.LY % (int)y == 0) { yb = y; break; }
+  }
+  Traversal t{};
+  if (want == 2 && L.T % (int)kXcds == 0 && yb <= (uint32_t)L.LY && L.LY % (int)yb == 0) {
+    t.mode = 2;
+    t.yb = yb;
+    t.runlen = yb * (uint32_t)L.LZ;
+    t.bpr = (t.runlen + S - 1) / S;
+    t.nrun = (uint32_t)(L.LY / (int)yb) * (uint32_t)L.LX;
+    t.nblk = (uint32_t)L.T * t.nrun * t.bpr;
+    return t;
+  }
+  if (slice % kXcds == 0) { t.csz = slice / kXcds; t.mode = 1; }
+  else if (V % kXcds == 0) { t.csz = V / kXcds; t.mode = 1; }
+  else { t.csz = V; t.mode = 0; }
+  if (want == 3) t.mode = 0;
   t.bpc = (t.csz + S - 1) / S;
   t.nblk = (V / t.csz) * t.bpc;
   return t;
@@ -340,7 +431,11 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   const Traversal tr = traversal(c, N);
   P.csz = tr.csz;
   P.bpc = tr.bpc;
-  P.xcd = tr.xcd;
+  P.mode = tr.mode;
+  P.yb = tr.yb;
+  P.runlen = tr.runlen;
+  P.bpr = tr.bpr;
+  P.nrun = tr.nrun;
   const unsigned nblk = tr.nblk;
   double *part = nullptr;
   if (dm != DOT_NONE) {

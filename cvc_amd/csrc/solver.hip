@@ -696,6 +696,15 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
       TRY(pull_state(S));
       int any_upd = 0;
       for (int r = 0; r < N; r++) any_upd |= (!S.hst[r].done && S.hst[r].upd);
+      if (verbose && (any_upd || it % 16 == 0)) {
+        double mx = 0.;
+        for (int r = 0; r < N; r++) {
+          if (S.hst[r].done) continue;
+          const double q = S.hst[r].norm / S.hst[r].normb;
+          if (!(q <= mx)) mx = q;   // a NaN propagates
+        }
+        printf("# [mixed] iteration %d: max iterated |r|^2/|b|^2 = %e\n", it, mx);
+      }
       if (any_upd) {
         // reliable update for all active rhs: x += xl; xl = 0; r = b - D64 x
         nrel++;
@@ -711,8 +720,10 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
         if (verbose) {
           TRY(pull_state(S));
           double mx = 0.;
-          for (int r = 0; r < N; r++)
-            if (S.hst[r].normb > 0 && S.hst[r].norm / S.hst[r].normb > mx) mx = S.hst[r].norm / S.hst[r].normb;
+          for (int r = 0; r < N; r++) {
+            const double q = S.hst[r].norm / S.hst[r].normb;
+            if (!(q <= mx)) mx = q;   // a NaN propagates
+          }
           printf("# [mixed] reliable update %d at iteration %d: max true |r|^2/|b|^2 = %e\n", nrel, it, mx);
           fflush(stdout);
         }

@@ -32,6 +32,7 @@ struct IOPlain<double> {
 #pragma unroll
     for (int c = 0; c < 12; c++) v[c] = p[(size_t)c * VN + idx];
   }
+  __device__ __forceinline__ C load_c(int c, uint32_t idx) const { return p[(size_t)c * VN + idx]; }
   __device__ __forceinline__ void store(const C v[12], uint32_t idx) const {
 #pragma unroll
     for (int c = 0; c < 12; c++) p[(size_t)c * VN + idx] = v[c];
@@ -90,8 +91,13 @@ struct IOHalf {
     float m = 0.f;
 #pragma unroll
     for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(v[c].x), fabsf(v[c].y)));
-    const float inv = m > 0.f ? 1.0f / m : 0.f;
-    nrm[idx] = m;
+    // a site whose largest component is below kTiny is stored as zero: its
+    // reciprocal scale would overflow FP32 (the far tail of a point-source
+    // Krylov vector reaches the denormal range on large lattices)
+    constexpr float kTiny = 1e-30f;
+    const bool keep = m >= kTiny;
+    const float inv = keep ? 1.0f / m : 0.f;
+    nrm[idx] = keep ? m : 0.f;
     uint4 *q = reinterpret_cast<uint4 *>(p);
 #pragma unroll
     for (int k = 0; k < 3; k++) {

@@ -64,6 +64,50 @@ def test_dslash_multi_rhs(chip, orc, nrhs):
             assert rel_err(got[r], L.Q_phi(g, phi[r])) < tol, (prec, r)
 
 
+# staging bits 0-1, traversal mode in bits 4-7, y-block in bits 8-15
+POLICIES = [0x00, 0x01, 0x03, 0x10, 0x11, 0x13, 0x21, 0x23, 0x221, 0x421, 0x30, 0x31]
+
+
+@pytest.mark.parametrize("nrhs", [1, 12])
+def test_dslash_policies(chip, orc, nrhs):
+    """Every traversal and staging policy gives the oracle's operator, and the
+    same bits as every other policy (the per-site arithmetic is the same)."""
+    dims = (16, 4, 8, 6)          # T divisible by 8: the slice-per-XCD traversal applies
+    print(f"dslash policies nrhs={nrhs}", flush=True)
+    g = chip.gauge_random(dims, 7, 0.3)
+    bc = (1.0, 0.0, 0.25, 0.0)
+    ctx = chip.Context(dims, 0.1632, 0.004, bc, gauge=g)
+    L = orc.Lattice(dims, 0.1632, 0.004, bc)
+    rng = np.random.default_rng(11 + nrhs)
+    phi = rand_spinor(rng, L.V, nrhs)
+    want = [L.Q_phi(g, phi[r]) for r in range(nrhs)]
+    fin, fout = ctx.field(nrhs), ctx.field(nrhs)
+    fin.upload(phi)
+    first = None
+    for pol in POLICIES:
+        ctx.set_dslash_policy(pol)
+        ctx.dslash(fout, fin)
+        got = fout.download()
+        for r in range(nrhs):
+            assert rel_err(got[r], want[r]) < 1e-13, (hex(pol), r)
+        if first is None:
+            first = got
+        else:
+            assert np.array_equal(got, first), hex(pol)
+        for prec, tol in [(chip.PREC_F32, 2e-6), (chip.PREC_F16, 2e-3)]:
+            a, b, back = ctx.field(nrhs, prec), ctx.field(nrhs, prec), ctx.field(nrhs)
+            a.copy_from(fin)
+            ctx.dslash(b, a)
+            back.copy_from(b)
+            lo = back.download()
+            for r in range(nrhs):
+                assert rel_err(lo[r], want[r]) < tol, (hex(pol), prec, r)
+    with pytest.raises(chip.CvchipError):
+        ctx.set_dslash_policy(0x321)      # y-block 3 does not divide LY = 8
+    with pytest.raises(chip.CvchipError):
+        ctx.set_dslash_policy(0x04)
+
+
 def test_scalar_products(chip, orc):
     dims = (4, 4, 4, 4)
     g = chip.gauge_random(dims, 1, 0.3)
