@@ -42,7 +42,35 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
     ap.add_argument("--delta16", type=float, default=0.1, help="reliable-update factor of the FP16 mixed solver")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device 3 times (the reference kernel for HBM counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_dslash_f64_12rhs_64x32x32x32_hbm_traffic.txt")
+
+
+def measured_traffic(dims, N):
+    """HBM bytes per launch of the 12-rhs FP64 operator at 64x32^3, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (line
+    'bytes_per_launch: B'), or None for any other workload."""
+    if tuple(dims) != (64, 32, 32, 32) or N != 12 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def copy_calibration():
+    import torch
+    x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 1 GiB
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": "3 x (1 GiB read + 1 GiB write)"}), flush=True)
 
 
 def log(msg):
@@ -55,6 +83,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design "
                                    "(source-level split) and is not executed", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
     import cvc_amd
@@ -161,7 +192,7 @@ def main():
                                f"(T,LX,LY,LZ), kappa={kappa}, mu={mu}, gauge exp(i 0.3 H) seed 4",
                    "lattice": list(dims), "nrhs": N, "link_recon": 18},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dims, N)},
         "cpu_baseline": cpu,
     }
     out.update(extra)

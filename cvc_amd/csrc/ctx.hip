@@ -251,12 +251,15 @@ extern "C" int cvchip_ctx_set_link_recon(cvchip_ctx *c, int recon) {
 extern "C" int cvchip_ctx_set_dslash_policy(cvchip_ctx *c, int policy) {
   if (!c) return CVCHIP_ERR_NULL;
   const int mode = (policy >> 4) & 15, yb = (policy >> 8) & 255;
-  if (policy < 0 || policy > 0xffff || (policy & 0xc) || mode > 3) {
-    set_error("operator policy 0x%x: bits 0-1 staging, 4-7 traversal mode 0..3, 8-15 y-block", policy);
+  const int bt = (policy >> 16) & 15, bx = (policy >> 20) & 15;
+  if (policy < 0 || policy >= (1 << 28) || mode > 4) {
+    set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..4, 8-15 y-block, "
+              "16-19 brick t, 20-23 brick x, 24-27 LDS padding", policy);
     return CVCHIP_ERR_ARG;
   }
-  if (yb && c->lat.LY % yb) {
-    set_error("operator policy: y-block %d does not divide LY = %d", yb, c->lat.LY);
+  if ((yb && c->lat.LY % yb) || (bt && c->lat.T % bt) || (bx && c->lat.LX % bx)) {
+    set_error("operator policy 0x%x: block extents must divide the lattice (T=%d LX=%d LY=%d)", policy,
+              c->lat.T, c->lat.LX, c->lat.LY);
     return CVCHIP_ERR_ARG;
   }
   c->dslash_policy = policy;

@@ -125,7 +125,7 @@ struct cvchip_ctx {
   int gauge_set = 0;
   int link_recon = 18;           // 18 or 12 for the FP64 Dslash
   // multi-rhs operator staging: bit 0 links in LDS, bit 1 own + z-neighbour spinors in LDS
-  int dslash_policy = 1;   // links staged in LDS, automatic traversal
+  int dslash_policy = 5;   // links staged in LDS, streaming stores, automatic traversal
   // staging for host <-> device transfers (AoS reference layout)
   double *stage = nullptr;
   size_t stage_bytes = 0;

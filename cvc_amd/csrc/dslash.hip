@@ -30,9 +30,14 @@ struct DParams {
   // sites), x within y block: the x and y neighbours of a run were touched a
   // few runs earlier and are still in the XCD's L2, and the t neighbours are
   // the slices the adjacent XCDs are working on at the same time.
+  // Mode 4: bricks of bt time slices x bx x-planes x yb y-rows (all z); the
+  // bricks of one (x, y) column follow each other in t on one XCD, so the
+  // sites in flight on an XCD form a compact 4D region.
   uint32_t csz, bpc;
   int mode;
   uint32_t yb, runlen, bpr, nrun;
+  uint32_t bt, bx, ncx, bpb, bpcol;
+  int nt;   // streaming output stores
   R k2, mutm;
   cplx<R> ph[4];
 };
@@ -225,7 +230,19 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
   constexpr int KL = RECON == 18 ? 9 : 6;
   const uint32_t S = blockDim.x / (uint32_t)P.N;
   uint32_t s0, s_end;
-  if (P.mode == 2) {
+  if (P.mode == 4) {
+    const uint32_t k = blockIdx.x / kXcds;
+    const uint32_t col = k / P.bpcol, rem = k - col * P.bpcol;
+    const uint32_t brick = rem / P.bpb, rem2 = rem - brick * P.bpb;
+    const uint32_t run = rem2 / P.bpr, blk = rem2 - run * P.bpr;
+    const uint32_t c = col * kXcds + blockIdx.x % kXcds;
+    const uint32_t cy = c / P.ncx, cx = c - cy * P.ncx;
+    const uint32_t tin = run / P.bx, xin = run - tin * P.bx;
+    const uint32_t t = brick * P.bt + tin, x = cx * P.bx + xin;
+    const uint32_t start = ((t * (uint32_t)P.LX + x) * (uint32_t)P.LY + cy * P.yb) * (uint32_t)P.LZ;
+    s0 = start + blk * S;
+    s_end = min(s0 + S, start + P.runlen);
+  } else if (P.mode == 2) {
     const uint32_t k = blockIdx.x / kXcds;
     const uint32_t per_slice = P.nrun * P.bpr;
     const uint32_t tl = k / per_slice, rem = k - tl * per_slice;
@@ -318,7 +335,8 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
       if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
       acc[c] = o;
     }
-    out.store(acc, i);
+    if (P.nt) out.store_nt(acc, i);
+    else out.store(acc, i);
     if constexpr (DM == DOT_TS) {
       // <out, in> = sum conj(out) in ; <out, out>   (or <out, r2> when a partner is given)
       if (P.partner) r2.load(own, i);
@@ -381,26 +399,50 @@ struct Traversal {
   uint32_t csz, bpc, nblk;
   int mode;
   uint32_t yb, runlen, bpr, nrun;
+  uint32_t bt, bx, ncx, bpb, bpcol;
 };
 
-// Block-to-site mapping; the policy's bits 4-7 pick the mode (0: automatic)
-// and bits 8-15 the y-block height of mode 2 (0: automatic).
+// Block-to-site mapping; the policy's bits 4-7 pick the mode (0: automatic),
+// bits 8-15 the y-block height of modes 2 and 4 (0: automatic), bits 16-19
+// and 20-23 the brick's t and x extents of mode 4 (0: 4).
 static Traversal traversal(const cvchip_ctx *c, int N) {
   const Lattice &L = c->lat;
   const uint32_t V = (uint32_t)L.V, slice = (uint32_t)(L.LX * L.LY * L.LZ);
   const uint32_t S = (uint32_t)(block_for(N) / N);
   int want = (c->dslash_policy >> 4) & 15;
   uint32_t yb = (uint32_t)((c->dslash_policy >> 8) & 255);
-  // automatic: slice per XCD in 16-row blocks for the multi-rhs operator
-  // (measured best at 64x32^3 with 12 rhs), XCD chunks of a slice otherwise
-  if (want == 0) want = (N >= 8 && L.T % (int)kXcds == 0) ? 2 : 1;
+  // automatic: bricks of 4 slices x 4 x-planes x 8 y-rows for the multi-rhs
+  // operator, else one slice per XCD in y blocks (both measured at 64x32^3
+  // with 12 rhs: 3.98 and 4.19 ms against 4.75 ms for XCD chunks of a
+  // slice); XCD chunks of a slice for a single rhs (0.48 ms against 0.54)
+  const bool autom = want == 0;
+  if (autom) want = N >= 8 ? 4 : 1;
   if (yb == 0) {
     yb = 1;
-    for (uint32_t y : {16u, 8u, 4u, 2u})
+    for (uint32_t y : {autom && want == 4 ? 8u : 16u, 8u, 4u, 2u})
       if (L.LY % (int)y == 0) { yb = y; break; }
   }
   Traversal t{};
-  if (want == 2 && L.T % (int)kXcds == 0 && yb <= (uint32_t)L.LY && L.LY % (int)yb == 0) {
+  uint32_t bt = (uint32_t)((c->dslash_policy >> 16) & 15), bx = (uint32_t)((c->dslash_policy >> 20) & 15);
+  if (bt == 0) bt = 4;
+  if (bx == 0) bx = 4;
+  if (want == 4 && L.T % (int)bt == 0 && L.LX % (int)bx == 0 && L.LY % (int)yb == 0 &&
+      ((uint32_t)(L.LX / (int)bx) * (uint32_t)(L.LY / (int)yb)) % kXcds == 0) {
+    t.mode = 4;
+    t.yb = yb;
+    t.bt = bt;
+    t.bx = bx;
+    t.ncx = (uint32_t)L.LX / bx;
+    t.runlen = yb * (uint32_t)L.LZ;
+    t.bpr = (t.runlen + S - 1) / S;
+    t.bpb = bt * bx * t.bpr;
+    t.bpcol = (uint32_t)L.T / bt * t.bpb;
+    const uint32_t ncol = t.ncx * ((uint32_t)L.LY / yb);
+    t.nblk = ncol * t.bpcol;
+    return t;
+  }
+  // (a brick request that does not tile the lattice falls back to mode 2)
+  if ((want == 2 || want == 4) && L.T % (int)kXcds == 0 && L.LY % (int)yb == 0) {
     t.mode = 2;
     t.yb = yb;
     t.runlen = yb * (uint32_t)L.LZ;
@@ -436,6 +478,14 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   P.runlen = tr.runlen;
   P.bpr = tr.bpr;
   P.nrun = tr.nrun;
+  P.bt = tr.bt;
+  P.bx = tr.bx;
+  P.ncx = tr.ncx;
+  P.bpb = tr.bpb;
+  P.bpcol = tr.bpcol;
+  // streaming stores only for outputs that cannot stay in the 256 MB MALL
+  // (bit 3: for every output)
+  P.nt = (((c->dslash_policy >> 2) & 1) && out.bytes >= ((size_t)256 << 20)) || ((c->dslash_policy >> 3) & 1);
   const unsigned nblk = tr.nblk;
   double *part = nullptr;
   if (dm != DOT_NONE) {
@@ -452,13 +502,15 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   const bool multi = B / N <= LDS_SITES;
   const bool ldsl = multi && (c->dslash_policy & 1);
   const bool ldss = multi && (c->dslash_policy & 2);
-  const size_t dyn = ldss ? (size_t)(B / N + 2) * N * 12 * sizeof(cplx<R>) : 0;
+  // bits 24-27: extra LDS per block in 8 KiB units (caps the blocks per CU)
+  const size_t pad = (size_t)((c->dslash_policy >> 24) & 15) * 8192;
+  const size_t dyn = (ldss ? (size_t)(B / N + 2) * N * 12 * sizeof(cplx<R>) : 0) + pad;
 #define CVC_LAUNCH(G5, DMV)                                                                                     \
   do {                                                                                                          \
     if (ldsl && ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, true>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
-    else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
+    else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
     else if (ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, true>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
-    else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false>), g, b, 0, c->stream, o, ii, U, P, rr, part); \
+    else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
   } while (0)
   if (g5) {
     if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);

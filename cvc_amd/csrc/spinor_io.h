@@ -37,6 +37,15 @@ struct IOPlain<double> {
 #pragma unroll
     for (int c = 0; c < 12; c++) p[(size_t)c * VN + idx] = v[c];
   }
+  // streaming store (no allocation in the caches)
+  __device__ __forceinline__ void store_nt(const C v[12], uint32_t idx) const {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      const d2v w = {v[c].x, v[c].y};
+      __builtin_nontemporal_store(w, reinterpret_cast<d2v *>(p + (size_t)c * VN + idx));
+    }
+  }
 };
 
 template <>
@@ -59,6 +68,15 @@ struct IOPlain<float> {
     float4 *q = reinterpret_cast<float4 *>(p);
 #pragma unroll
     for (int k = 0; k < 6; k++) q[(size_t)k * VN + idx] = make_float4(v[2 * k].x, v[2 * k].y, v[2 * k + 1].x, v[2 * k + 1].y);
+  }
+  __device__ __forceinline__ void store_nt(const C v[12], uint32_t idx) const {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v *q = reinterpret_cast<f4v *>(p);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const f4v w = {v[2 * k].x, v[2 * k].y, v[2 * k + 1].x, v[2 * k + 1].y};
+      __builtin_nontemporal_store(w, q + (size_t)k * VN + idx);
+    }
   }
 };
 
@@ -87,7 +105,10 @@ struct IOHalf {
       }
     }
   }
-  __device__ __forceinline__ void store(const C v[12], uint32_t idx) const {
+  __device__ __forceinline__ void store(const C v[12], uint32_t idx) const { store_t<false>(v, idx); }
+  __device__ __forceinline__ void store_nt(const C v[12], uint32_t idx) const { store_t<true>(v, idx); }
+  template <bool NT>
+  __device__ __forceinline__ void store_t(const C v[12], uint32_t idx) const {
     float m = 0.f;
 #pragma unroll
     for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(v[c].x), fabsf(v[c].y)));
@@ -97,8 +118,10 @@ struct IOHalf {
     constexpr float kTiny = 1e-30f;
     const bool keep = m >= kTiny;
     const float inv = keep ? 1.0f / m : 0.f;
-    nrm[idx] = keep ? m : 0.f;
-    uint4 *q = reinterpret_cast<uint4 *>(p);
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    if constexpr (NT) __builtin_nontemporal_store(keep ? m : 0.f, nrm + idx);
+    else nrm[idx] = keep ? m : 0.f;
+    u4v *q = reinterpret_cast<u4v *>(p);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       uint32_t u[4];
@@ -107,7 +130,9 @@ struct IOHalf {
         const __half2 h = __floats2half2_rn(v[4 * k + j].x * inv, v[4 * k + j].y * inv);
         __builtin_memcpy(&u[j], &h, 4);
       }
-      q[(size_t)k * VN + idx] = make_uint4(u[0], u[1], u[2], u[3]);
+      const u4v w = {u[0], u[1], u[2], u[3]};
+      if constexpr (NT) __builtin_nontemporal_store(w, q + (size_t)k * VN + idx);
+      else q[(size_t)k * VN + idx] = w;
     }
   }
 };

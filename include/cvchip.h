@@ -67,11 +67,16 @@ int cvchip_ctx_set_params(cvchip_ctx *ctx, double kappa, double mu);
 /* link storage of the FP64 operator: 18 (full) or 12 (two rows, third rebuilt) */
 int cvchip_ctx_set_link_recon(cvchip_ctx *ctx, int recon);
 /* operator tuning.  Bits 0-1, staging of the multi-rhs operator (N >= 8): bit 0
- * keeps a block's links in LDS, bit 1 its own and z-neighbour spinors.  Bits
- * 4-7, block-to-site traversal: 0 automatic, 1 time slices cut into 8 XCD
- * chunks, 2 one time slice per XCD walked in y blocks, 3 linear.  Bits 8-15:
- * the y-block height of traversal 2 (0: the largest of 16, 8, 4, 2 dividing
- * LY).  Default 1: links in LDS, automatic traversal.  The operator output does not
+ * keeps a block's links in LDS, bit 1 its own and z-neighbour spinors; bit 2
+ * writes outputs of 256 MB and more with streaming (non-temporal) stores, bit 3
+ * every output.  Bits 4-7,
+ * block-to-site traversal: 0 automatic, 1 time slices cut into 8 XCD chunks,
+ * 2 one time slice per XCD walked in y blocks, 3 linear, 4 bricks of bt x bx x
+ * yb x LZ sites marched along t per XCD.  Bits 8-15: the y-block height of
+ * traversals 2 and 4 (0: the largest of 16, 8, 4, 2 dividing LY); bits 16-19
+ * bt and 20-23 bx of traversal 4 (0: 4); bits 24-27 extra LDS per block in
+ * 8 KiB units (caps the resident blocks).  Default 5: links in LDS,
+ * streaming stores, automatic traversal (bricks 4x4x8 for N >= 8).  The operator output does not
  * depend on it; the fused dot products of the solvers differ at rounding level. */
 int cvchip_ctx_set_dslash_policy(cvchip_ctx *ctx, int policy);
 const char *cvchip_last_error(void);
