@@ -12,10 +12,12 @@ its algorithmic HBM throughput: per site 12 x (192 B in + 192 B out) + 576 B
 of links = 5184 B, divided by the average launch duration from HIP events on
 the library's stream.  Beside it the line carries the 12-rhs solve times to a
 relative residual of 1e-10 (squared 1e-20) in FP64 and mixed precision, the
-single-rhs operator figures, and the oracle's CPU operator timed on a bounded
-sample (one rhs) on this host.
+time of one whole source (120 propagators and the Pi_mu_nu(x) contraction),
+the single-rhs operator figures, and the oracle's CPU operator timed on a
+bounded sample (one rhs) on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--quick]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-source] [--no-cpu]
+       python bench.py --copy-calibration   (reference copy kernel for the HBM counters)
 """
 from __future__ import annotations
 
@@ -40,6 +42,8 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--no-solve", action="store_true", help="operator timing only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-source", action="store_true",
+                    help="skip the whole-source run (120 propagators, FP16+norm inner solves, contraction)")
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
     ap.add_argument("--delta16", type=float, default=0.1, help="reliable-update factor of the FP16 mixed solver")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -141,13 +145,16 @@ def main():
     if not a.no_solve:
         b = ctx.field(N)
         b.upload(np.stack([cvc_amd.point_source(V, 0, k // 3, k % 3) for k in range(N)]))
-        for prec, name in ((cvc_amd.PREC_F64, "f64"), (cvc_amd.PREC_F32, "mixed_f32"),
-                           (cvc_amd.PREC_F16, "mixed_f16")):
+        # the reference's algorithm (unpreconditioned), then even/odd preconditioned
+        for eo, prec, name in ((False, cvc_amd.PREC_F64, "f64"), (False, cvc_amd.PREC_F32, "mixed_f32"),
+                               (False, cvc_amd.PREC_F16, "mixed_f16"), (True, cvc_amd.PREC_F64, "eo_f64"),
+                               (True, cvc_amd.PREC_F32, "eo_mixed_f32"), (True, cvc_amd.PREC_F16, "eo_mixed_f16")):
             x = ctx.field(N)
             torch.cuda.synchronize()
             s0 = time.perf_counter()
             rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
-                                    delta=a.delta16 if prec == cvc_amd.PREC_F16 else a.delta, check=False)
+                                    delta=a.delta16 if prec == cvc_amd.PREC_F16 else a.delta, check=False,
+                                    eo=eo)
             torch.cuda.synchronize()
             el = time.perf_counter() - s0
             extra[f"solve_{name}_s"] = round(el, 3)
@@ -161,6 +168,25 @@ def main():
             extra["mixed_f32_speedup"] = round(extra["solve_f64_s"] / extra["solve_mixed_f32_s"], 3)
         if extra.get("solve_mixed_f16_s"):
             extra["mixed_f16_speedup"] = round(extra["solve_f64_s"] / extra["solve_mixed_f16_s"], 3)
+        for k in ("eo_f64", "eo_mixed_f32", "eo_mixed_f16"):
+            if extra.get(f"solve_{k}_s"):
+                extra[f"{k}_speedup"] = round(extra["solve_f64_s"] / extra[f"solve_{k}_s"], 3)
+
+    if not a.no_solve and not a.no_source:
+        # one source point: 60 up + 60 down propagators (10 batches of 12, even/odd
+        # preconditioned, FP16+norm inner, FP64 reliable updates) and the Pi_mu_nu(x) contraction
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        ctx.set_solver_flags(cvc_amd.SOLVE_EO)
+        conn, ct, st = ctx.cvc_source((0, 0, 0, 0), 1e-20, 4000, cvc_amd.PREC_F16, a.delta16)
+        ctx.set_solver_flags(0)
+        el = time.perf_counter() - s0
+        extra["source_solve_contract_s"] = round(el, 3)
+        extra["source_solves_s"] = round(st["seconds"], 3)
+        extra["source_converged"] = st["converged"]
+        extra["source_true_res2_max"] = st["true_res2_max"]
+        log(f"whole source (120 solves FP16+norm inner + contraction): {el:.2f}s {st}")
+        del conn
 
     cpu = None
     if not a.no_cpu:

@@ -16,12 +16,13 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcvchip.so")
 
 PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
-SOLVE_LAGGED, SOLVE_VERBOSE = 1, 2
+SOLVE_LAGGED, SOLVE_VERBOSE, SOLVE_EO = 1, 2, 4
 ERR_NULL, ERR_NOCONV = -2, -3
 
 # every entry point declared in include/cvchip.h
 EXPORTS = [
     "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
+    "cvchip_ctx_set_solver_flags",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
@@ -65,6 +66,7 @@ def lib():
         L.cvchip_ctx_destroy.argtypes = [_vp]
         L.cvchip_ctx_set_params.argtypes = [_vp, C.c_double, C.c_double]
         L.cvchip_ctx_set_link_recon.argtypes = [_vp, C.c_int]
+        L.cvchip_ctx_set_solver_flags.argtypes = [_vp, C.c_uint]
         L.cvchip_ctx_set_dslash_policy.argtypes = [_vp, C.c_int]
         L.cvchip_set_gauge.argtypes = [_vp, _dp]
         L.cvchip_Q_phi.argtypes = [_vp, _dp, _dp, C.c_double, C.c_int]
@@ -200,6 +202,10 @@ class Context:
     def set_link_recon(self, recon: int):
         _check(lib().cvchip_ctx_set_link_recon(self._h, int(recon)))
 
+    def set_solver_flags(self, flags: int):
+        """Flags OR-ed into every solve of this context (e.g. SOLVE_EO)."""
+        _check(lib().cvchip_ctx_set_solver_flags(self._h, int(flags)))
+
     def set_dslash_policy(self, policy: int):
         _check(lib().cvchip_ctx_set_dslash_policy(self._h, int(policy)))
 
@@ -229,12 +235,12 @@ class Context:
         return float(w[0])
 
     def invert_Qtm(self, b, sp, niter_max=1000, x0=None, prec=PREC_F64, delta=0.1, lagged=False,
-                   verbose=False, check=True):
+                   verbose=False, check=True, eo=False):
         b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, self.V, 24)
         n = b.shape[0]
         x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, order="C").reshape(b.shape)
         st = Stats()
-        flags = (SOLVE_LAGGED if lagged else 0) | (SOLVE_VERBOSE if verbose else 0)
+        flags = (SOLVE_LAGGED if lagged else 0) | (SOLVE_VERBOSE if verbose else 0) | (SOLVE_EO if eo else 0)
         rc = lib().cvchip_invert_Qtm(self._h, _ptr(x), _ptr(b), n, float(sp), int(niter_max), int(prec),
                                      float(delta), flags, C.byref(st))
         if check and rc < 0:
@@ -242,13 +248,14 @@ class Context:
         return rc, x, st.as_dict()
 
     def invert_Qtm_her(self, b, sp, niter_max=1000, x0=None, prec=PREC_F64, delta=0.1, lagged=False,
-                       check=True):
+                       check=True, eo=False):
         b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, self.V, 24)
         n = b.shape[0]
         x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64, order="C").reshape(b.shape)
         st = Stats()
         rc = lib().cvchip_invert_Qtm_her(self._h, _ptr(x), _ptr(b), n, float(sp), int(niter_max), int(prec),
-                                         float(delta), SOLVE_LAGGED if lagged else 0, C.byref(st))
+                                         float(delta), (SOLVE_LAGGED if lagged else 0) | (SOLVE_EO if eo else 0),
+                                         C.byref(st))
         if check and rc < 0:
             raise CvchipError(rc, last_error())
         return rc, x, st.as_dict()
@@ -268,19 +275,20 @@ class Context:
         return float(ms[0])
 
     def invert(self, x: Field, b: Field, sp, niter_max=1000, prec=PREC_F64, delta=0.1, lagged=False,
-               check=True):
+               check=True, eo=False):
         st = Stats()
         rc = lib().cvchip_field_invert(self._h, x._h, b._h, float(sp), int(niter_max), int(prec), float(delta),
-                                       SOLVE_LAGGED if lagged else 0, C.byref(st))
+                                       (SOLVE_LAGGED if lagged else 0) | (SOLVE_EO if eo else 0), C.byref(st))
         if check and rc < 0:
             raise CvchipError(rc, last_error())
         return rc, st.as_dict()
 
     def invert_her(self, x: Field, b: Field, sp, niter_max=1000, prec=PREC_F64, delta=0.1, check=True,
-                   verbose=False):
+                   verbose=False, eo=False):
         st = Stats()
         rc = lib().cvchip_field_invert_her(self._h, x._h, b._h, float(sp), int(niter_max), int(prec),
-                                           float(delta), SOLVE_VERBOSE if verbose else 0, C.byref(st))
+                                           float(delta), (SOLVE_VERBOSE if verbose else 0) | (SOLVE_EO if eo else 0),
+                                           C.byref(st))
         if check and rc < 0:
             raise CvchipError(rc, last_error())
         return rc, st.as_dict()

@@ -48,7 +48,7 @@ static size_t elem_bytes(int prec) {
   return prec == CVCHIP_PREC_F64 ? sizeof(double2) : prec == CVCHIP_PREC_F32 ? sizeof(float2) : sizeof(__half2);
 }
 
-int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec) {
+int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites) {
   if (nrhs <= 0 || nrhs > 256) {
     set_error("nrhs %d outside 1..256", nrhs);
     return CVCHIP_ERR_ARG;
@@ -63,7 +63,7 @@ int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec) {
   }
   f.nrhs = nrhs;
   f.prec = prec;
-  f.V = c->lat.V;
+  f.V = sites > 0 ? sites : c->lat.V;
   f.bytes = (size_t)12 * f.V * nrhs * elem_bytes(prec);
   CVC_HIP(hipMalloc(&f.data, f.bytes));
   if (prec == CVCHIP_PREC_F16) CVC_HIP(hipMalloc(&f.norm, sizeof(float) * f.V * nrhs));
@@ -245,6 +245,20 @@ extern "C" int cvchip_ctx_set_link_recon(cvchip_ctx *c, int recon) {
     return CVCHIP_ERR_ARG;
   }
   c->link_recon = recon;
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_ctx_set_solver_flags(cvchip_ctx *c, unsigned flags) {
+  if (!c) return CVCHIP_ERR_NULL;
+  if (flags & ~(CVCHIP_SOLVE_LAGGED | CVCHIP_SOLVE_VERBOSE | CVCHIP_SOLVE_EO)) {
+    set_error("unknown solver flags 0x%x", flags);
+    return CVCHIP_ERR_ARG;
+  }
+  if ((flags & CVCHIP_SOLVE_EO) && c->lat.LZ % 2) {
+    set_error("even/odd preconditioning needs an even LZ (got %d)", c->lat.LZ);
+    return CVCHIP_ERR_ARG;
+  }
+  c->solve_flags = flags;
   return CVCHIP_OK;
 }
 

@@ -126,6 +126,7 @@ struct cvchip_ctx {
   int link_recon = 18;           // 18 or 12 for the FP64 Dslash
   // multi-rhs operator staging: bit 0 links in LDS, bit 1 own + z-neighbour spinors in LDS
   int dslash_policy = 5;   // links staged in LDS, streaming stores, automatic traversal
+  unsigned solve_flags = 0;   // OR-ed into every solve
   // staging for host <-> device transfers (AoS reference layout)
   double *stage = nullptr;
   size_t stage_bytes = 0;
@@ -138,7 +139,8 @@ namespace cvchip {
 
 int ensure_stage(cvchip_ctx *c, size_t bytes);
 int ensure_red(cvchip_ctx *c, size_t bytes);
-int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec);
+// sites: 0 for the whole lattice, or the half lattice V/2 of the even/odd solver
+int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites = 0);
 void field_free(Field &f);
 
 // host AoS [V][24] (nrhs contiguous fields) <-> device field rhs range
@@ -153,7 +155,16 @@ enum DotMode { DOT_NONE = 0, DOT_TS = 1, DOT_R2 = 2 };
 int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_out,
            DotMode dm = DOT_NONE, const Field *r2 = nullptr, int *nblocks_out = nullptr);
 
+// Even/odd pieces on half-lattice fields (output parity par):
+//   dmode 0: A own + hopc H in     1: A^-1 (hopc H in)
+//   dmode 2: own + hopc H in       3: A^-1 (own + hopc H in)
+// with A = 1/(2 kappa) + i mutm gamma5, H the hopping sum (D = A - H/2).
+int dslash_eo(cvchip_ctx *c, Field &out, const Field &in, const Field *own, double mutm, int par, int dmode,
+              double hopc, int gamma5_out, DotMode dm = DOT_NONE, const Field *r2 = nullptr,
+              int *nblocks_out = nullptr);
+
 // number of blocks (= reduction partial slots) of an operator launch on N rhs
+// (the larger of the whole-lattice and the half-lattice traversal)
 unsigned dslash_blocks(const cvchip_ctx *c, int N);
 
 // block-size convention shared by all per-rhs reductions

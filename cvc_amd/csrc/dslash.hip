@@ -38,6 +38,16 @@ struct DParams {
   uint32_t yb, runlen, bpr, nrun;
   uint32_t bt, bx, ncx, bpb, bpcol;
   int nt;   // streaming output stores
+  // Even/odd operator (EO kernels): the traversal runs over the half lattice
+  // (T, LX, LY, LZ/2); half-site h of parity par is the lexicographic site
+  // with z = 2 (h mod LZ/2) + ((par + t + x + y) mod 2), i.e. lex = 2h or 2h+1.
+  // The epilogue forms, with A = 1/(2 kappa) + i mutm gamma5 on the output
+  // parity and hop the hopping sum:
+  //   dmode 0: A own + hopc hop     1: A^-1 (hopc hop)
+  //   dmode 2: own + hopc hop       3: A^-1 (own + hopc hop)
+  uint32_t LZt;   // z extent of the traversal (LZ, or LZ/2 for EO)
+  int par, dmode;
+  R hopc, dinv;   // dinv = 1 / ((1/(2 kappa))^2 + mutm^2)
   R k2, mutm;
   cplx<R> ph[4];
 };
@@ -181,7 +191,7 @@ __device__ __forceinline__ void pin_regs(C *v, int n) {
   for (int k = 0; k < n; k++) asm volatile("" : "+v"(v[k].x), "+v"(v[k].y) : : "memory");
 }
 
-template <typename IO, int RECON, int MU, bool LDSL, bool LDSS>
+template <typename IO, int RECON, int MU, bool LDSL, bool LDSS, bool EO>
 __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const NbLoader<IO, LDSS> &ld,
                                          const cplx<typename IO::real> *__restrict__ U,
                                          const cplx<typename IO::real> *__restrict__ Ul,
@@ -193,13 +203,13 @@ __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const 
   const uint32_t sm = coord == 0 ? s + (uint32_t)(L - 1) * stride : s - stride;
   const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
   // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
-  ld.template proj<MU, false>(h, sm, MU == 3);
+  ld.template proj<MU, false>(h, EO ? sm >> 1 : sm, MU == 3);
   if constexpr (LDSL && CVC_HOP_PIN > 1) pin_regs(&h[0][0], 6);
   if constexpr (LDSL) link_apply<R, RECON, MU, false>(acc, h, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
   else link_apply<R, RECON, MU, false>(acc, h, U + (size_t)MU * KL * P.V + sm, P.V, P.ph[MU]);
   if constexpr (LDSL && CVC_HOP_PIN > 0) pin_regs(acc, 12);
   // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
-  ld.template proj<MU, true>(h, sp, MU == 3);
+  ld.template proj<MU, true>(h, EO ? sp >> 1 : sp, MU == 3);
   if constexpr (LDSL && CVC_HOP_PIN > 1) pin_regs(&h[0][0], 6);
   if constexpr (LDSL) link_apply<R, RECON, MU, true>(acc, h, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
   else link_apply<R, RECON, MU, true>(acc, h, U + (size_t)MU * KL * P.V + s, P.V, P.ph[MU]);
@@ -221,13 +231,27 @@ template <typename R> struct DslashWaves {
 // LDSL: the block's sites share their links through LDS (N >= 8, at most
 // LDS_SITES sites per block): every link entry crosses the texture path once
 // per block instead of once per rhs lane.
-template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL, bool LDSS>
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL, bool LDSS, bool EO>
 __global__ void __launch_bounds__(256, DslashWaves<typename IOI::real>::value)
-k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParams<typename IOI::real> P,
-         IOI r2, double *__restrict__ part) {
+k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict__ U,
+         DParams<typename IOI::real> P, IOI r2, double *__restrict__ part) {
   using R = typename IOI::real;
   using C = cplx<R>;
   constexpr int KL = RECON == 18 ? 9 : 6;
+  static_assert(!(EO && LDSS), "the even/odd operator stages no spinors");
+  // lexicographic site of traversal index v
+  auto lex_of = [&](uint32_t v) -> uint32_t {
+    if constexpr (!EO) {
+      return v;
+    } else {
+      const uint32_t zl = v % P.LZt;
+      uint32_t q = v / P.LZt;
+      const uint32_t y = q % (uint32_t)P.LY;
+      q /= (uint32_t)P.LY;
+      const uint32_t x = q % (uint32_t)P.LX, t = q / (uint32_t)P.LX;
+      return ((q * (uint32_t)P.LY + y) * (uint32_t)P.LZ) + 2u * zl + (((uint32_t)P.par + t + x + y) & 1u);
+    }
+  };
   const uint32_t S = blockDim.x / (uint32_t)P.N;
   uint32_t s0, s_end;
   if (P.mode == 4) {
@@ -239,7 +263,7 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     const uint32_t cy = c / P.ncx, cx = c - cy * P.ncx;
     const uint32_t tin = run / P.bx, xin = run - tin * P.bx;
     const uint32_t t = brick * P.bt + tin, x = cx * P.bx + xin;
-    const uint32_t start = ((t * (uint32_t)P.LX + x) * (uint32_t)P.LY + cy * P.yb) * (uint32_t)P.LZ;
+    const uint32_t start = ((t * (uint32_t)P.LX + x) * (uint32_t)P.LY + cy * P.yb) * P.LZt;
     s0 = start + blk * S;
     s_end = min(s0 + S, start + P.runlen);
   } else if (P.mode == 2) {
@@ -249,7 +273,7 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     const uint32_t run = rem / P.bpr, blk = rem - run * P.bpr;
     const uint32_t t = tl * kXcds + blockIdx.x % kXcds;
     const uint32_t ybl = run / (uint32_t)P.LX, x = run - ybl * (uint32_t)P.LX;
-    const uint32_t start = ((t * (uint32_t)P.LX + x) * (uint32_t)P.LY + ybl * P.yb) * (uint32_t)P.LZ;
+    const uint32_t start = ((t * (uint32_t)P.LX + x) * (uint32_t)P.LY + ybl * P.yb) * P.LZt;
     s0 = start + blk * S;
     s_end = min(s0 + S, start + P.runlen);
   } else {
@@ -266,8 +290,8 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     s_end = min(s0 + S, (chunk + 1) * P.csz);
   }
   const uint32_t r = threadIdx.x % (uint32_t)P.N;
-  const uint32_t s = s0 + threadIdx.x / (uint32_t)P.N;
-  const uint32_t i = s * (uint32_t)P.N + r;
+  const uint32_t vs = s0 + threadIdx.x / (uint32_t)P.N;   // traversal index (site, or half-site for EO)
+  const uint32_t i = vs * (uint32_t)P.N + r;
   double dv[3] = {0., 0., 0.};
   __shared__ C Ul[LDSL ? 8 * KL * LDS_SITES : 1];
   if constexpr (LDSL) {
@@ -275,8 +299,8 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     for (uint32_t e = threadIdx.x; e < tot; e += blockDim.x) {
       const uint32_t hopi = e / (KL * S), rem = e - hopi * (KL * S);
       const uint32_t k = rem / S, sl = rem - k * S;
-      const uint32_t site = s0 + sl;
-      if (site >= s_end) continue;
+      if (s0 + sl >= s_end) continue;
+      const uint32_t site = lex_of(s0 + sl);
       const int mu = (int)(hopi >> 1);
       uint32_t ls = site;
       if ((hopi & 1u) == 0) {          // backward hop: link at site - mu
@@ -305,7 +329,8 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     __syncthreads();
   }
   const NbLoader<IOI, LDSS> ld{in, Sp, (uint32_t)P.N, r, s0, S};
-  if (s < s_end) {
+  if (vs < s_end) {
+    const uint32_t s = lex_of(vs);
     const uint32_t t = s / P.sT;
     uint32_t rem = s - t * P.sT;
     const uint32_t x = rem / P.sX;
@@ -316,24 +341,49 @@ k_dslash(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParam
     C acc[12];
 #pragma unroll
     for (int c = 0; c < 12; c++) acc[c] = mk<R>(0, 0);
-    const uint32_t sl = s - s0;
-    hop_pair<IOI, RECON, 0, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)t, P.T, P.sT);
-    hop_pair<IOI, RECON, 1, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)x, P.LX, P.sX);
-    hop_pair<IOI, RECON, 2, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)y, P.LY, (uint32_t)P.LZ);
-    hop_pair<IOI, RECON, 3, LDSL, LDSS>(acc, ld, U, Ul, P, s, sl, (int)z, P.LZ, 1u);
+    const uint32_t sl = vs - s0;
+    hop_pair<IOI, RECON, 0, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)t, P.T, P.sT);
+    hop_pair<IOI, RECON, 1, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)x, P.LX, P.sX);
+    hop_pair<IOI, RECON, 2, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)y, P.LY, (uint32_t)P.LZ);
+    hop_pair<IOI, RECON, 3, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)z, P.LZ, 1u);
 
     C own[12];
-    ld.full(own, s, true);
-    const R mh = (R)-0.5;
+    if constexpr (!EO) {
+      ld.full(own, s, true);
+      const R mh = (R)-0.5;
 #pragma unroll
-    for (int c = 0; c < 12; c++) {
-      // -1/2 hop + phi/(2 kappa) + i mutm gamma5 phi  (gamma5 = +1 on spins 0,1, -1 on 2,3)
-      const R m = c < 6 ? P.mutm : -P.mutm;
-      C o;
-      o.x = mh * acc[c].x + P.k2 * own[c].x - m * own[c].y;
-      o.y = mh * acc[c].y + P.k2 * own[c].y + m * own[c].x;
-      if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
-      acc[c] = o;
+      for (int c = 0; c < 12; c++) {
+        // -1/2 hop + phi/(2 kappa) + i mutm gamma5 phi  (gamma5 = +1 on spins 0,1, -1 on 2,3)
+        const R m = c < 6 ? P.mutm : -P.mutm;
+        C o;
+        o.x = mh * acc[c].x + P.k2 * own[c].x - m * own[c].y;
+        o.y = mh * acc[c].y + P.k2 * own[c].y + m * own[c].x;
+        if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
+        acc[c] = o;
+      }
+    } else {
+      if (P.dmode != 1) own_f.load(own, i);
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        const R m = c < 6 ? P.mutm : -P.mutm;
+        C h = mk<R>(P.hopc * acc[c].x, P.hopc * acc[c].y);
+        C o;
+        if (P.dmode == 0) {
+          o.x = h.x + P.k2 * own[c].x - m * own[c].y;
+          o.y = h.y + P.k2 * own[c].y + m * own[c].x;
+        } else {
+          if (P.dmode >= 2) h = cadd(h, own[c]);
+          if (P.dmode == 2) {
+            o = h;
+          } else {
+            // A^-1 h = (1/(2 kappa) - i mutm gamma5) h / ((1/(2 kappa))^2 + mutm^2)
+            o.x = (P.k2 * h.x + m * h.y) * P.dinv;
+            o.y = (P.k2 * h.y - m * h.x) * P.dinv;
+          }
+        }
+        if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
+        acc[c] = o;
+      }
     }
     if (P.nt) out.store_nt(acc, i);
     else out.store(acc, i);
@@ -405,8 +455,12 @@ struct Traversal {
 // Block-to-site mapping; the policy's bits 4-7 pick the mode (0: automatic),
 // bits 8-15 the y-block height of modes 2 and 4 (0: automatic), bits 16-19
 // and 20-23 the brick's t and x extents of mode 4 (0: 4).
-static Traversal traversal(const cvchip_ctx *c, int N) {
-  const Lattice &L = c->lat;
+static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
+  Lattice L = c->lat;
+  if (eo) {   // the half lattice of one parity: LZ/2 half-sites per z line
+    L.LZ /= 2;
+    L.V /= 2;
+  }
   const uint32_t V = (uint32_t)L.V, slice = (uint32_t)(L.LX * L.LY * L.LZ);
   const uint32_t S = (uint32_t)(block_for(N) / N);
   int want = (c->dslash_policy >> 4) & 15;
@@ -460,17 +514,32 @@ static Traversal traversal(const cvchip_ctx *c, int N) {
   return t;
 }
 
-unsigned dslash_blocks(const cvchip_ctx *c, int N) { return traversal(c, N).nblk; }
+unsigned dslash_blocks(const cvchip_ctx *c, int N) {
+  const unsigned a = traversal(c, N).nblk;
+  const unsigned b = c->lat.LZ % 2 == 0 ? traversal(c, N, true).nblk : 0u;
+  return a > b ? a : b;
+}
 
-template <typename IOI, typename IOO, int RECON>
+struct EOArgs {
+  const Field *own;
+  int par, dmode;
+  double hopc;
+};
+
+template <typename IOI, typename IOO, int RECON, bool EO>
 static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g5, DotMode dm,
-                  const Field *r2, int *nblk_out) {
+                  const Field *r2, int *nblk_out, const EOArgs *eo = nullptr) {
   using R = typename IOI::real;
   const int N = in.nrhs;
   DParams<R> P = make_params<R>(c, N, mutm);
   P.partner = (dm == DOT_TS && r2 != nullptr) ? 1 : 0;
+  P.LZt = EO ? (uint32_t)c->lat.LZ / 2 : (uint32_t)c->lat.LZ;
+  P.par = eo ? eo->par : 0;
+  P.dmode = eo ? eo->dmode : 0;
+  P.hopc = (R)(eo ? eo->hopc : -0.5);
+  P.dinv = (R)(1.0 / (0.25 / (c->kappa * c->kappa) + mutm * mutm));
   const int B = block_for(N);
-  const Traversal tr = traversal(c, N);
+  const Traversal tr = traversal(c, N, EO);
   P.csz = tr.csz;
   P.bpc = tr.bpc;
   P.mode = tr.mode;
@@ -498,19 +567,25 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   IOO o = make_io<IOO>(out);
   IOI ii = make_io<IOI>(in);
   IOI rr = r2 ? make_io<IOI>(*r2) : ii;
+  IOI ow = (eo && eo->own) ? make_io<IOI>(*eo->own) : ii;
   dim3 g(nblk), b(B);
   const bool multi = B / N <= LDS_SITES;
   const bool ldsl = multi && (c->dslash_policy & 1);
-  const bool ldss = multi && (c->dslash_policy & 2);
+  const bool ldss = !EO && multi && (c->dslash_policy & 2);
   // bits 24-27: extra LDS per block in 8 KiB units (caps the blocks per CU)
   const size_t pad = (size_t)((c->dslash_policy >> 24) & 15) * 8192;
   const size_t dyn = (ldss ? (size_t)(B / N + 2) * N * 12 * sizeof(cplx<R>) : 0) + pad;
 #define CVC_LAUNCH(G5, DMV)                                                                                     \
   do {                                                                                                          \
-    if (ldsl && ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, true>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
-    else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
-    else if (ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, true>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
-    else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false>), g, b, dyn, c->stream, o, ii, U, P, rr, part); \
+    if constexpr (EO) {                                                                                         \
+      if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false, true>), g, b, dyn, c->stream, o, ii, ow, U, P, rr, part); \
+      else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false, true>), g, b, dyn, c->stream, o, ii, ow, U, P, rr, part); \
+    } else {                                                                                                    \
+      if (ldsl && ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, true, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      else if (ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, true, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+    }                                                                                                           \
   } while (0)
   if (g5) {
     if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);
@@ -543,15 +618,53 @@ int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_o
   }
   if (in.prec == CVCHIP_PREC_F64 && out.prec == CVCHIP_PREC_F64) {
     if (c->link_recon == 12)
-      return launch<IOPlain<double>, IOPlain<double>, 12>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
-    return launch<IOPlain<double>, IOPlain<double>, 18>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+      return launch<IOPlain<double>, IOPlain<double>, 12, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+    return launch<IOPlain<double>, IOPlain<double>, 18, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
   }
   if (in.prec == CVCHIP_PREC_F32 && out.prec == CVCHIP_PREC_F32)
-    return launch<IOPlain<float>, IOPlain<float>, 12>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+    return launch<IOPlain<float>, IOPlain<float>, 12, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
   if (in.prec == CVCHIP_PREC_F16 && out.prec == CVCHIP_PREC_F16)
-    return launch<IOHalf, IOHalf, 12>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+    return launch<IOHalf, IOHalf, 12, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
   set_error("unsupported operator precision pair %d -> %d", in.prec, out.prec);
   return CVCHIP_ERR_ARG;
+}
+
+// Even/odd operator pieces on half-lattice fields (V/2 sites, half-site order
+// of DParams): out_par = dmode form of (own, hopc x hopping sum of in), in at
+// the opposite parity.  With H the hopping sum and A = 1/(2 kappa) + i mutm g5,
+// D = A - H/2 splits as D_ee = A, D_eo = -H_eo/2 (Q_phi.c:58-174 restricted to
+// one parity of the output).
+int dslash_eo(cvchip_ctx *c, Field &out, const Field &in, const Field *own, double mutm, int par, int dmode,
+              double hopc, int gamma5_out, DotMode dm, const Field *r2, int *nblocks_out) {
+  if (!c->gauge_set) {
+    set_error("gauge field not set (call cvchip_set_gauge first)");
+    return CVCHIP_ERR_STATE;
+  }
+  if (c->lat.LZ % 2) {
+    set_error("even/odd operator needs an even LZ (got %d)", c->lat.LZ);
+    return CVCHIP_ERR_ARG;
+  }
+  const int64_t Vh = c->lat.V / 2;
+  if (out.V != Vh || in.V != Vh || out.nrhs != in.nrhs || out.prec != in.prec ||
+      (dmode != 1 && (!own || own->V != Vh || own->nrhs != in.nrhs || own->prec != in.prec)) ||
+      (r2 && (r2->V != Vh || r2->nrhs != in.nrhs || r2->prec != in.prec)) || dmode < 0 || dmode > 3 ||
+      (par != 0 && par != 1)) {
+    set_error("even/odd operator: fields must be half-lattice sets of one size and precision");
+    return CVCHIP_ERR_ARG;
+  }
+  if (out.data == in.data) {
+    set_error("operator output must not alias its input");
+    return CVCHIP_ERR_ARG;
+  }
+  const EOArgs eo{own, par, dmode, hopc};
+  if (in.prec == CVCHIP_PREC_F64) {
+    if (c->link_recon == 12)
+      return launch<IOPlain<double>, IOPlain<double>, 12, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
+    return launch<IOPlain<double>, IOPlain<double>, 18, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
+  }
+  if (in.prec == CVCHIP_PREC_F32)
+    return launch<IOPlain<float>, IOPlain<float>, 12, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
+  return launch<IOHalf, IOHalf, 12, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
 }
 
 // ------------------------------------------------------- reductions

@@ -416,10 +416,11 @@ struct Solve {
   RhsState *hst = nullptr;    // pinned host mirror
 };
 
-static int solve_setup(cvchip_ctx *c, Solve &S, int N) {
+// V: sites of the solver's vectors (the lattice, or one parity of it)
+static int solve_setup(cvchip_ctx *c, Solve &S, int N, int64_t V = 0) {
   S.c = c;
   S.N = N;
-  S.VN = (uint32_t)(c->lat.V * N);
+  S.VN = (uint32_t)((V > 0 ? V : c->lat.V) * N);
   S.B = block_for(N);
   S.nblk = (S.VN + S.B - 1) / S.B;
   // room for the operator's partials too (its traversal has its own block count)
@@ -484,7 +485,22 @@ struct Op {
   cvchip_ctx *c;
   int her;
   Field *aux;
+  int eo = 0;
+  Field *tmp = nullptr;   // even/odd: the even-parity intermediate
+  // Schur complement on the odd sites, M = A_oo - D_oe A_ee^-1 D_eo
+  //   = A_oo - H_oe A_ee^-1 H_eo / 4  (gamma5 on the output when g5); the fused
+  // <out,in> of DOT_TS pairs the output with `partner` (default: in)
+  int applyM(Field &out, const Field &in, double mutm, int g5, DotMode dm, const Field *partner, int *nb) {
+    CVC_TRY(dslash_eo(c, *tmp, in, nullptr, mutm, 0, 1, 0.5, 0));
+    return dslash_eo(c, out, *tmp, &in, mutm, 1, 0, -0.5, g5, dm,
+                     dm == DOT_TS ? (partner ? partner : &in) : partner, nb);
+  }
   int apply(Field &out, const Field &in, DotMode dm, const Field *r2, int *nb) {
+    if (eo) {
+      if (!her) return applyM(out, in, c->mu, 0, dm, r2, nb);
+      CVC_TRY(applyM(*aux, in, -c->mu, 1, DOT_NONE, nullptr, nullptr));
+      return applyM(out, *aux, c->mu, 1, dm, dm == DOT_TS ? &in : r2, nb);
+    }
     if (!her) return dslash(c, out, in, c->mu, 0, dm, r2, nb);
     CVC_TRY(dslash(c, *aux, in, -c->mu, 1));
     // the fused <t,s> must pair the output with the operator's input, not with aux
@@ -508,18 +524,19 @@ __global__ void k_g5(double2 *__restrict__ f, uint32_t VN) {
 // FP64 BiCGStab (on the gamma5-hermitian system when her != 0; then b is
 // expected as gamma5 b and x receives y of A y = gamma5 b)
 static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, unsigned flags,
-                      cvchip_stats *stats, int her = 0) {
+                      cvchip_stats *stats, int her = 0, int eo = 0) {
   const int N = b.nrhs;
   Solve S;
-  int rc = solve_setup(c, S, N);
-  Field r1, r2, s, t, p, p2, aux;
-  Field *all[7] = {&r1, &r2, &s, &t, &p, &p2, &aux};
-  for (int k = 0; k < 6 + (her ? 1 : 0) && !rc; k++) rc = field_alloc(c, *all[k], N, CVCHIP_PREC_F64);
+  int rc = solve_setup(c, S, N, x.V);
+  Field r1, r2, s, t, p, p2, aux, tmp;
+  Field *all[8] = {&r1, &r2, &s, &t, &p, &p2, &aux, &tmp};
+  for (int k = 0; k < 6 + (her ? 1 : 0) && !rc; k++) rc = field_alloc(c, *all[k], N, CVCHIP_PREC_F64, x.V);
+  if (!rc && eo) rc = field_alloc(c, tmp, N, CVCHIP_PREC_F64, x.V);
   auto cleanup = [&]() {
-    for (int k = 0; k < 7; k++) field_free(*all[k]);
+    for (int k = 0; k < 8; k++) field_free(*all[k]);
     solve_free(S);
   };
-  Op op{c, her, &aux};
+  Op op{c, her, &aux, eo, &tmp};
   if (rc) {
     cleanup();
     return rc;
@@ -592,7 +609,7 @@ static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
     }
   }
   double tr = 0.;
-  if (!her) {
+  if (!her && !eo) {
     TRY(true_residual(S, x, b, t, c->mu, &tr, nullptr));
     dcalls++;
   }
@@ -617,26 +634,31 @@ static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
 // gamma5-hermitian system when her != 0, b then being gamma5 b).
 template <typename IOL>
 static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec,
-                          double delta, cvchip_stats *stats, int her = 0, int verbose = 0) {
+                          double delta, cvchip_stats *stats, int her = 0, int verbose = 0, int eo = 0) {
   const int N = b.nrhs;
   Solve S;
-  int rc = solve_setup(c, S, N);
-  Field r1, r2, s, t, p, p2, xl, t64, auxl, aux64;
+  int rc = solve_setup(c, S, N, x.V);
+  Field r1, r2, s, t, p, p2, xl, t64, auxl, aux64, tmpl, tmp64;
   Field *lo[7] = {&r1, &r2, &s, &t, &p, &p2, &xl};
-  for (int k = 0; k < 6 && !rc; k++) rc = field_alloc(c, *lo[k], N, prec);
+  const int64_t V = x.V;
+  for (int k = 0; k < 6 && !rc; k++) rc = field_alloc(c, *lo[k], N, prec, V);
   // the solution accumulator since the last reliable update stays FP32
-  if (!rc) rc = field_alloc(c, xl, N, CVCHIP_PREC_F32);
-  if (!rc) rc = field_alloc(c, t64, N, CVCHIP_PREC_F64);
-  if (!rc && her) rc = field_alloc(c, auxl, N, prec);
-  if (!rc && her) rc = field_alloc(c, aux64, N, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, xl, N, CVCHIP_PREC_F32, V);
+  if (!rc) rc = field_alloc(c, t64, N, CVCHIP_PREC_F64, V);
+  if (!rc && her) rc = field_alloc(c, auxl, N, prec, V);
+  if (!rc && her) rc = field_alloc(c, aux64, N, CVCHIP_PREC_F64, V);
+  if (!rc && eo) rc = field_alloc(c, tmpl, N, prec, V);
+  if (!rc && eo) rc = field_alloc(c, tmp64, N, CVCHIP_PREC_F64, V);
   auto cleanup = [&]() {
     for (int k = 0; k < 7; k++) field_free(*lo[k]);
     field_free(t64);
     field_free(auxl);
     field_free(aux64);
+    field_free(tmpl);
+    field_free(tmp64);
     solve_free(S);
   };
-  Op opl{c, her, &auxl}, op64{c, her, &aux64};
+  Op opl{c, her, &auxl, eo, &tmpl}, op64{c, her, &aux64, eo, &tmp64};
   // FP16 storage keeps too few digits for the Krylov space to survive a
   // residual replacement: restart the recurrence at every reliable update
   const bool restart = prec == CVCHIP_PREC_F16;
@@ -758,7 +780,7 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
     }
   // fold in what the low-precision accumulator still holds (non-converged rhs)
   double tr = 0.;
-  if (!her) {
+  if (!her && !eo) {
     TRY(true_residual(S, x, b, t64, c->mu, &tr, nullptr));
     dcalls++;
   }
@@ -779,12 +801,165 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
   return maxit;
 }
 
+// ------------------------------------------------------------ even/odd
+
+// FP64 set <-> its two parity halves (half-site h of parity p is the
+// lexicographic site 2h or 2h+1 of that parity: z = 2 (h mod LZ/2) + ...)
+__global__ void __launch_bounds__(256) k_eo_split(double2 *__restrict__ full, double2 *__restrict__ ev,
+                                                  double2 *__restrict__ od, uint32_t V, int N, int LX, int LY,
+                                                  int LZ, int to_full) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t VN = V * (uint32_t)N;
+  if (i >= VN) return;
+  const uint32_t site = i / (uint32_t)N, r = i - site * (uint32_t)N;
+  const uint32_t z = site % (uint32_t)LZ;
+  uint32_t q = site / (uint32_t)LZ;
+  const uint32_t y = q % (uint32_t)LY;
+  q /= (uint32_t)LY;
+  const uint32_t x = q % (uint32_t)LX, t = q / (uint32_t)LX;
+  double2 *h = ((t + x + y + z) & 1u) ? od : ev;
+  const uint32_t j = (site >> 1) * (uint32_t)N + r, VhN = VN / 2;
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    if (to_full) full[(size_t)c * VN + i] = h[(size_t)c * VhN + j];
+    else h[(size_t)c * VhN + j] = full[(size_t)c * VN + i];
+  }
+}
+
+static int eo_split(cvchip_ctx *c, Field &full, Field &ev, Field &od, int to_full) {
+  const uint32_t VN = (uint32_t)(full.V * full.nrhs);
+  hipLaunchKernelGGL(k_eo_split, dim3((VN + 255) / 256), dim3(256), 0, c->stream, (double2 *)full.data,
+                     (double2 *)ev.data, (double2 *)od.data, (uint32_t)full.V, full.nrhs, c->lat.LX, c->lat.LY,
+                     c->lat.LZ, to_full);
+  CVC_HIP(hipGetLastError());
+  return 0;
+}
+
+// out = A^-1 in on one parity, A = 1/(2 kappa) + i mutm gamma5 (FP64)
+__global__ void __launch_bounds__(256) k_diag_inv(double2 *__restrict__ out, const double2 *__restrict__ in,
+                                                  uint32_t VN, double k2, double mutm, double dinv) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= VN) return;
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    const double m = c < 6 ? mutm : -mutm;
+    const double2 v = in[(size_t)c * VN + i];
+    out[(size_t)c * VN + i] = make_double2((k2 * v.x + m * v.y) * dinv, (k2 * v.y - m * v.x) * dinv);
+  }
+}
+
+// D(mu) x = b through the Schur complement on the odd sites:
+//   M x_o = b_o - D_oe A_ee^-1 b_e = b_o + H_oe A_ee^-1 b_e / 2,
+//   x_e = A_ee^-1 (b_e - D_eo x_o) = A_ee^-1 (b_e + H_eo x_o / 2),
+// M = A_oo - H_oe A_ee^-1 H_eo / 4.  her: M is solved like the full operator,
+// as gamma5 M(mu) gamma5 M(-mu) y = gamma5 b'_o, x_o = gamma5 M(-mu) y
+// (M(mu)^dagger = gamma5 M(-mu) gamma5).  The convergence test is the
+// reference's, on the odd system; the returned statistics carry the true
+// residual |D x - b|^2 / |b|^2 of the whole lattice.
+static int invert_eo(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec, double delta,
+                     unsigned flags, cvchip_stats *stats, int her) {
+  if (c->lat.LZ % 2) {
+    set_error("even/odd preconditioning needs an even LZ (got %d)", c->lat.LZ);
+    return CVCHIP_ERR_ARG;
+  }
+  if (prec != CVCHIP_PREC_F64 && !(delta > 0. && delta < 1.)) {
+    set_error("reliable-update factor delta must lie in (0,1), got %g", delta);
+    return CVCHIP_ERR_ARG;
+  }
+  if (prec < CVCHIP_PREC_F64 || prec > CVCHIP_PREC_F16) {
+    set_error("unknown precision %d", prec);
+    return CVCHIP_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const int N = b.nrhs;
+  const int64_t Vh = c->lat.V / 2;
+  const double mu = c->mu;
+  Field be, bo, xe, xo, bp, y;
+  int rc = 0;
+  Field *hf[6] = {&be, &bo, &xe, &xo, &bp, &y};
+  for (int k = 0; k < 5 + (her ? 1 : 0) && !rc; k++) rc = field_alloc(c, *hf[k], N, CVCHIP_PREC_F64, Vh);
+  auto cleanup = [&]() {
+    for (int k = 0; k < 6; k++) field_free(*hf[k]);
+  };
+#define TRY(x)                                                                                   \
+  do {                                                                                           \
+    rc = (x);                                                                                    \
+    if (rc) {                                                                                    \
+      cleanup();                                                                                 \
+      return rc;                                                                                 \
+    }                                                                                            \
+  } while (0)
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  TRY(eo_split(c, const_cast<Field &>(b), be, bo, 0));
+  TRY(eo_split(c, x, xe, xo, 0));
+  const uint32_t VhN = (uint32_t)(Vh * N);
+  const double k2 = 0.5 / c->kappa, dinv = 1.0 / (k2 * k2 + mu * mu);
+  hipLaunchKernelGGL(k_diag_inv, dim3((VhN + 255) / 256), dim3(256), 0, c->stream, (double2 *)xe.data,
+                     (const double2 *)be.data, VhN, k2, mu, dinv);
+  TRY(dslash_eo(c, bp, xe, &bo, mu, 1, 2, 0.5, 0));      // b'_o = b_o + H_oe A_ee^-1 b_e / 2
+  int ret;
+  if (her) {
+    hipLaunchKernelGGL(k_g5, dim3((VhN + 255) / 256), dim3(256), 0, c->stream, (double2 *)bp.data, VhN);
+    CVC_HIP(hipMemcpyAsync(y.data, xo.data, xo.bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (prec == CVCHIP_PREC_F64) ret = bicgstab64(c, y, bp, sp, niter_max, flags, stats, 1, 1);
+    else if (prec == CVCHIP_PREC_F32)
+      ret = bicgstab_mixed<IOPlain<float>>(c, y, bp, sp, niter_max, prec, delta, stats, 1,
+                                           (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0, 1);
+    else
+      ret = bicgstab_mixed<IOHalf>(c, y, bp, sp, niter_max, prec, delta, stats, 1,
+                                   (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0, 1);
+    if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
+      Op m{c, 0, nullptr, 1, &xe};
+      TRY(m.applyM(xo, y, -mu, 1, DOT_NONE, nullptr, nullptr));    // x_o = gamma5 M(-mu) y
+    }
+  } else {
+    if (prec == CVCHIP_PREC_F64) ret = bicgstab64(c, xo, bp, sp, niter_max, flags, stats, 0, 1);
+    else if (prec == CVCHIP_PREC_F32)
+      ret = bicgstab_mixed<IOPlain<float>>(c, xo, bp, sp, niter_max, prec, delta, stats, 0,
+                                           (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0, 1);
+    else
+      ret = bicgstab_mixed<IOHalf>(c, xo, bp, sp, niter_max, prec, delta, stats, 0,
+                                   (flags & CVCHIP_SOLVE_VERBOSE) ? 1 : 0, 1);
+  }
+  if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
+    TRY(dslash_eo(c, xe, xo, &be, mu, 0, 3, 0.5, 0));    // x_e = A_ee^-1 (b_e + H_eo x_o / 2)
+    TRY(eo_split(c, x, xe, xo, 1));
+    if (stats) {
+      Solve S;
+      Field tmp;
+      double tr = 0.;
+      int rc3 = solve_setup(c, S, N);
+      if (!rc3) rc3 = field_alloc(c, tmp, N, CVCHIP_PREC_F64);
+      if (!rc3) rc3 = true_residual(S, x, b, tmp, mu, &tr, nullptr);
+      field_free(tmp);
+      solve_free(S);
+      if (rc3) ret = rc3;
+      stats->true_res2_max = tr;
+      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  }
+  cleanup();
+#undef TRY
+  return ret;
+}
+
 // invert_Qtm_her (invert_Qtm.c:276-478): y from A y = gamma5 b, then x = gamma5 D(-mu) y.
 int invert_her_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_max, int prec,
                       double delta, unsigned flags, cvchip_stats *stats) {
   if (!c->gauge_set) {
     set_error("gauge field not set (call cvchip_set_gauge first)");
     return CVCHIP_ERR_STATE;
+  }
+  flags |= c->solve_flags;
+  if (flags & CVCHIP_SOLVE_EO) {
+    if (niter_max < 0 || !(sp >= 0.)) {
+      set_error("invalid niter_max %d or tolerance %g", niter_max, sp);
+      return CVCHIP_ERR_ARG;
+    }
+    return invert_eo(c, x, b, sp, niter_max, prec, delta, flags, stats, 1);
   }
   auto t0 = std::chrono::steady_clock::now();
   Field g5b, y;
@@ -849,6 +1024,8 @@ int invert_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int niter_
     set_error("invalid niter_max %d or tolerance %g", niter_max, sp);
     return CVCHIP_ERR_ARG;
   }
+  flags |= c->solve_flags;
+  if (flags & CVCHIP_SOLVE_EO) return invert_eo(c, x, b, sp, niter_max, prec, delta, flags, stats, 0);
   auto t0 = std::chrono::steady_clock::now();
   int rc;
   if (prec == CVCHIP_PREC_F64) rc = bicgstab64(c, x, b, sp, niter_max, flags, stats);

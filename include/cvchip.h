@@ -43,6 +43,11 @@ extern "C" {
 #define CVCHIP_SOLVE_LAGGED   1u  /* return the iterate from before the last update,
                                      as invert_Qtm.c:199-211 does */
 #define CVCHIP_SOLVE_VERBOSE  2u  /* print the reference's per-iteration residual lines */
+#define CVCHIP_SOLVE_EO       4u  /* even/odd preconditioning: the solver runs on the Schur
+                                     complement of the odd sites (half-lattice vectors) and
+                                     rebuilds the even sites; same solution and true-residual
+                                     check, fewer and cheaper iterations than the reference's
+                                     unpreconditioned BiCGStab (needs an even LZ) */
 
 typedef struct cvchip_ctx cvchip_ctx;
 typedef struct cvchip_field cvchip_field;
@@ -66,6 +71,10 @@ int cvchip_ctx_destroy(cvchip_ctx *ctx);
 int cvchip_ctx_set_params(cvchip_ctx *ctx, double kappa, double mu);
 /* link storage of the FP64 operator: 18 (full) or 12 (two rows, third rebuilt) */
 int cvchip_ctx_set_link_recon(cvchip_ctx *ctx, int recon);
+/* solver flags OR-ed into every solve of the context, also those inside
+ * cvchip_cvc_source and cvchip_cvc_gradient (for instance CVCHIP_SOLVE_EO).
+ * No reference counterpart; default 0. */
+int cvchip_ctx_set_solver_flags(cvchip_ctx *ctx, unsigned flags);
 /* operator tuning.  Bits 0-1, staging of the multi-rhs operator (N >= 8): bit 0
  * keeps a block's links in LDS, bit 1 its own and z-neighbour spinors; bit 2
  * writes outputs of 256 MB and more with streaming (non-temporal) stores, bit 3

@@ -40,3 +40,43 @@ def test_contraction_16x32(chip, orc):
     assert np.max(np.abs(ct - ct_o)) < 1e-10 * np.max(np.abs(ct_o))
     piq, piq_o = ctx.pi_q(conn, ct, src), ctx.pi_q(conn_o, ct_o, src)
     assert rel_err(piq, piq_o) < 1e-10
+
+
+def piq_errors(piq, ref, dims, qmax=2.0):
+    """Per-momentum error of Pi_mu_nu(q) for every q with 0 < |q| <= qmax: max over
+    (mu, nu) of |Pi - Pi_ref| over the largest |Pi_ref_mu_nu(q)| of that q.  Also the
+    error at q = 0, where the Ward identity makes Pi(0) vanish up to rounding (so
+    it is measured against the largest |Pi_ref(q)| of all q instead)."""
+    z = (piq[..., 0] + 1j * piq[..., 1]).reshape(dims + (16,))
+    zr = (ref[..., 0] + 1j * ref[..., 1]).reshape(dims + (16,))
+    n = np.stack(np.meshgrid(*[np.fft.fftfreq(L, 1.0 / L) for L in dims], indexing="ij"), -1)
+    qq = ((2 * np.pi * n / np.array(dims)) ** 2).sum(-1)
+    sel = (qq <= qmax * qmax + 1e-12) & (qq > 0)
+    err = np.abs(z - zr).max(-1)[sel] / np.abs(zr).max(-1)[sel]
+    err0 = np.abs(z - zr)[(0,) * 4].max() / np.abs(zr).max()
+    return err, int(sel.sum()), err0
+
+
+@pytest.mark.timeout(900)
+def test_piq_mixed_precision_24x48(chip):
+    """24^3x48, kappa 0.1632, mu 0.004, gauge seed 3: Pi_mu_nu(q) from 120 propagators
+    solved with FP32 and FP16+norm inner solvers (FP64 reliable updates) against the
+    FP64 solves, per q with |q| <= 2, all to |r|^2/|b|^2 <= 1e-20.  The FP64 GPU path
+    is the oracle's equal on the same propagators (test_contraction_16x32)."""
+    dims = (48, 24, 24, 24)
+    src = (0, 0, 0, 0)
+    print("24^3x48 Pi(q) mixed vs FP64", flush=True)
+    g = chip.gauge_random(dims, 3, 0.3)
+    ctx = chip.Context(dims, 0.1632, 0.004, (1.0, 0.0, 0.0, 0.0), gauge=g)
+    res = {}
+    for prec, delta in ((chip.PREC_F64, 0.1), (chip.PREC_F32, 0.01), (chip.PREC_F16, 0.1)):
+        conn, ct, st = ctx.cvc_source(src, 1e-20, 4000, prec, delta)
+        res[prec] = ctx.pi_q(conn, ct, src)
+        print(f"prec {prec}: {st['seconds']:.2f} s of solves, {st['iterations']} iterations summed, "
+              f"{st['reliable_updates']} reliable updates, max true |r|^2/|b|^2 {st['true_res2_max']:.2e}", flush=True)
+    for prec in (chip.PREC_F32, chip.PREC_F16):
+        err, nq, err0 = piq_errors(res[prec], res[chip.PREC_F64], dims)
+        print(f"prec {prec}: per-q relative error of Pi_mu_nu(q) over {nq} momenta 0 < |q| <= 2: "
+              f"max {err.max():.2e}, median {np.median(err):.2e}; q = 0: {err0:.2e} of max |Pi|", flush=True)
+        assert err.max() < 1e-7
+        assert err0 < 1e-9
