@@ -44,6 +44,16 @@ int ensure_red(cvchip_ctx *c, size_t bytes) {
   return 0;
 }
 
+int ensure_red2(cvchip_ctx *c, size_t bytes) {
+  if (c->red2.bytes >= bytes) return 0;
+  if (c->red2.ptr) CVC_HIP(hipFree(c->red2.ptr));
+  c->red2.ptr = nullptr;
+  c->red2.bytes = 0;
+  CVC_HIP(hipMalloc(&c->red2.ptr, bytes));
+  c->red2.bytes = bytes;
+  return 0;
+}
+
 static size_t elem_bytes(int prec) {
   return prec == CVCHIP_PREC_F64 ? sizeof(double2) : prec == CVCHIP_PREC_F32 ? sizeof(float2) : sizeof(__half2);
 }
@@ -220,6 +230,7 @@ extern "C" int cvchip_ctx_destroy(cvchip_ctx *c) {
   if (c->gauge32r12) (void)hipFree(c->gauge32r12);
   if (c->stage) (void)hipFree(c->stage);
   if (c->red.ptr) (void)hipFree(c->red.ptr);
+  if (c->red2.ptr) (void)hipFree(c->red2.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);

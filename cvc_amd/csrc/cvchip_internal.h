@@ -132,6 +132,7 @@ struct cvchip_ctx {
   size_t stage_bytes = 0;
   // reduction scratch
   cvchip::Scratch red;
+  cvchip::Scratch red2;   // first-stage sums of the two-stage partial reduction
   // per-context solver workspaces are allocated per call
 };
 
@@ -139,6 +140,7 @@ namespace cvchip {
 
 int ensure_stage(cvchip_ctx *c, size_t bytes);
 int ensure_red(cvchip_ctx *c, size_t bytes);
+int ensure_red2(cvchip_ctx *c, size_t bytes);
 // sites: 0 for the whole lattice, or the half lattice V/2 of the even/odd solver
 int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites = 0);
 void field_free(Field &f);

@@ -669,6 +669,34 @@ int dslash_eo(cvchip_ctx *c, Field &out, const Field &in, const Field *own, doub
 
 // ------------------------------------------------------- reductions
 
+// Two fixed-order stages: G blocks per (q, rhs) row each sum a contiguous
+// segment of the row (four independent accumulators per thread, so several
+// loads are in flight), then one block per row sums the G segment totals.
+__global__ void __launch_bounds__(256) k_reduce_seg(const double *__restrict__ part, int nblk, int G,
+                                                    double *__restrict__ seg) {
+  __shared__ double sh[256];
+  const int row = blockIdx.x / G, g = blockIdx.x - row * G;
+  const int len = (nblk + G - 1) / G;
+  const int b0 = g * len, b1 = min(nblk, b0 + len);
+  const double *p = part + (size_t)row * nblk;
+  double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+  int b = b0 + (int)threadIdx.x;
+  for (; b + 768 < b1; b += 1024) {
+    a0 += p[b];
+    a1 += p[b + 256];
+    a2 += p[b + 512];
+    a3 += p[b + 768];
+  }
+  for (; b < b1; b += 256) a0 += p[b];
+  sh[threadIdx.x] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) seg[blockIdx.x] = sh[0];
+}
+
 __global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int Q, int N,
                                   double *__restrict__ res) {
   __shared__ double sh[256];
@@ -687,7 +715,18 @@ __global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int
 }
 
 int reduce_partials(cvchip_ctx *c, const double *part, int nblk, int Q, int N, double *res) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(Q * N), dim3(256), 0, c->stream, part, nblk, Q, N, res);
+  const int rows = Q * N;
+  // segments of at least 4096 partials, at most 64 per row
+  int G = nblk / 4096;
+  if (G > 64) G = 64;
+  if (G >= 2) {
+    CVC_TRY(ensure_red2(c, sizeof(double) * (size_t)rows * G));
+    double *seg = (double *)c->red2.ptr;
+    hipLaunchKernelGGL(k_reduce_seg, dim3(rows * G), dim3(256), 0, c->stream, part, nblk, G, seg);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(rows), dim3(256), 0, c->stream, seg, G, Q, N, res);
+  } else {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(rows), dim3(256), 0, c->stream, part, nblk, Q, N, res);
+  }
   CVC_HIP(hipGetLastError());
   return 0;
 }

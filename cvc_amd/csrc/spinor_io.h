@@ -167,8 +167,9 @@ __device__ __forceinline__ void block_reduce_store(const double v[Q], double *__
   }
 }
 
-// Reduces part[Q][N][nblk] over blocks into res[q*N + rhs]; launched with one
-// block of 256 threads per (q, rhs); fixed order, so results are reproducible.
+// Reduces part[Q][N][nblk] over blocks into res[q*N + rhs] (reduce_partials:
+// segment sums, then one block of 256 threads per (q, rhs)); fixed order, so
+// results are reproducible.
 __global__ void k_reduce_partials(const double *__restrict__ part, int nblk, int Q, int N,
                                   double *__restrict__ res);
 
