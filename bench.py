@@ -140,6 +140,16 @@ def main():
         extra[f"dslash_{N}rhs_{name}_ms"] = round(mp, 5)
         extra[f"dslash_{N}rhs_{name}_GBs"] = round((N * 2 * sb + lb) * V / (mp * 1e-3) / 1e9, 1)
         del pi, po
+    # 48 rhs (12 spin-colour x 4 source positions) in one launch, links shared by all 48
+    if N == 12:
+        f48i, f48o = ctx.field(48), ctx.field(48)
+        f48i.fill_random(3)
+        for _ in range(2):
+            ctx.dslash(f48o, f48i)
+        m48 = ctx.dslash_timed(f48o, f48i, max(10, a.steps // 2))
+        extra["dslash_48rhs_f64_ms"] = round(m48, 5)
+        extra["dslash_48rhs_f64_GBs"] = round((48 * 384 + 576) * V / (m48 * 1e-3) / 1e9, 1)
+        del f48i, f48o
     log(f"operator variants: {extra}")
 
     if not a.no_solve:

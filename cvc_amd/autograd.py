@@ -34,13 +34,22 @@ def momenta(dims, qmax):
     return q[(q * q).sum(1) <= qmax * qmax + 1e-12]
 
 
+def momentum_weight(dims, src, qmax):
+    """c(x) = sum_{q in Q} cos q.(x - y) for all sites x (flat, lexicographic), and |Q|,
+    by one inverse FFT of the indicator of Q on the momentum grid."""
+    n = np.meshgrid(*[np.fft.fftfreq(L, 1.0 / L) for L in dims], indexing="ij")
+    qq = sum((2 * np.pi * nd / L) ** 2 for nd, L in zip(n, dims))
+    mask = (qq <= qmax * qmax + 1e-12).astype(np.float64)
+    w = np.fft.ifftn(mask).real * mask.size           # w[d] = sum_q cos(q.d)
+    w = np.roll(w, shift=tuple(src), axis=(0, 1, 2, 3))   # w[x] -> sum_q cos(q.(x - y))
+    return w.reshape(-1), int(mask.sum())
+
+
 def loss_from_conn(conn, contact, dims, src, qmax):
     """L = sum_mu sum_{q in Q} Re Pi_{mu mu}(q) from conn[V,16,2] and the contact term."""
-    q = momenta(dims, qmax)
-    X = np.stack(np.meshgrid(*[np.arange(L) for L in dims], indexing="ij"), -1).reshape(-1, 4)
-    w = np.cos((X - np.array(src)) @ q.T).sum(1)
+    w, nq = momentum_weight(dims, src, qmax)
     diag = conn[:, [0, 5, 10, 15], 0].sum(1)
-    return float((w * diag).sum() - len(q) * np.real(np.sum(contact)))
+    return float((w * diag).sum() - nq * np.real(np.sum(contact)))
 
 
 def expi(H):
