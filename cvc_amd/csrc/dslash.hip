@@ -342,10 +342,20 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
 #pragma unroll
     for (int c = 0; c < 12; c++) acc[c] = mk<R>(0, 0);
     const uint32_t sl = vs - s0;
-    hop_pair<IOI, RECON, 0, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)t, P.T, P.sT);
-    hop_pair<IOI, RECON, 1, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)x, P.LX, P.sX);
-    hop_pair<IOI, RECON, 2, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)y, P.LY, (uint32_t)P.LZ);
-    hop_pair<IOI, RECON, 3, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)z, P.LZ, 1u);
+#ifndef CVC_HOP_ORDER
+#define CVC_HOP_ORDER 0
+#endif
+#define CVC_HOP(MU, CRD, L, STR) hop_pair<IOI, RECON, MU, LDSL, LDSS, EO>(acc, ld, U, Ul, P, s, sl, (int)(CRD), L, STR)
+#if CVC_HOP_ORDER == 0
+    CVC_HOP(0, t, P.T, P.sT); CVC_HOP(1, x, P.LX, P.sX); CVC_HOP(2, y, P.LY, (uint32_t)P.LZ); CVC_HOP(3, z, P.LZ, 1u);
+#elif CVC_HOP_ORDER == 1
+    CVC_HOP(1, x, P.LX, P.sX); CVC_HOP(2, y, P.LY, (uint32_t)P.LZ); CVC_HOP(3, z, P.LZ, 1u); CVC_HOP(0, t, P.T, P.sT);
+#elif CVC_HOP_ORDER == 2
+    CVC_HOP(0, t, P.T, P.sT); CVC_HOP(3, z, P.LZ, 1u); CVC_HOP(2, y, P.LY, (uint32_t)P.LZ); CVC_HOP(1, x, P.LX, P.sX);
+#else
+    CVC_HOP(0, t, P.T, P.sT); CVC_HOP(1, x, P.LX, P.sX); CVC_HOP(3, z, P.LZ, 1u); CVC_HOP(2, y, P.LY, (uint32_t)P.LZ);
+#endif
+#undef CVC_HOP
 
     C own[12];
     if constexpr (!EO) {
