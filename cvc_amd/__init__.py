@@ -22,7 +22,7 @@ ERR_NULL, ERR_NOCONV = -2, -3
 # every entry point declared in include/cvchip.h
 EXPORTS = [
     "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
-    "cvchip_ctx_set_solver_flags",
+    "cvchip_ctx_set_solver_flags", "cvchip_piq_orbits",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
@@ -97,6 +97,8 @@ def lib():
         L.cvchip_cvc_gradient.argtypes = [_vp, C.POINTER(C.c_int), C.c_double, C.c_double, C.c_int, C.c_int,
                                           C.c_double, _dp, _dp, _dp, C.POINTER(Stats)]
         L.cvchip_gauge_random.argtypes = [C.POINTER(C.c_int), C.c_uint64, C.c_double, _dp]
+        L.cvchip_piq_orbits.argtypes = [C.POINTER(C.c_int), _dp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -124,6 +126,22 @@ def gauge_random(dims, seed, eps=0.3) -> np.ndarray:
     g = np.empty((V, 4, 18))
     _check(lib().cvchip_gauge_random((C.c_int * 4)(*dims), C.c_uint64(int(seed)), float(eps), _ptr(g)))
     return g
+
+
+def piq_orbits(dims, piq, max_orbits=1 << 20):
+    """Scalar vacuum polarisation Pi(qhat^2) = -sum_mu Re Pi_mumu(q) / (3 qhat^2) averaged on
+    orbits of equal qhat^2 (q = 0 left out): returns qhat2, pi, pi_sd, count (arrays)."""
+    dims = [int(d) for d in dims]
+    piq = np.ascontiguousarray(piq, dtype=np.float64)
+    assert piq.shape == (int(np.prod(dims)), 16, 2)
+    m = min(int(max_orbits), int(np.prod(dims)))
+    q2, pi, sd = np.empty(m), np.empty(m), np.empty(m)
+    cnt = np.empty(m, dtype=np.int32)
+    no = C.c_int(0)
+    _check(lib().cvchip_piq_orbits((C.c_int * 4)(*dims), _ptr(piq), m, _ptr(q2), _ptr(pi), _ptr(sd),
+                                   cnt.ctypes.data_as(C.POINTER(C.c_int)), C.byref(no)))
+    k = no.value
+    return q2[:k], pi[:k], sd[:k], cnt[:k]
 
 
 def point_source(V, site, spin, colour) -> np.ndarray:

@@ -204,13 +204,13 @@ __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const 
   const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
   // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
   ld.template proj<MU, false>(h, EO ? sm >> 1 : sm, MU == 3);
-  if constexpr (LDSL && CVC_HOP_PIN > 1) pin_regs(&h[0][0], 6);
+  if constexpr (LDSL && CVC_HOP_PIN == 2) pin_regs(&h[0][0], 6);
   if constexpr (LDSL) link_apply<R, RECON, MU, false>(acc, h, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
   else link_apply<R, RECON, MU, false>(acc, h, U + (size_t)MU * KL * P.V + sm, P.V, P.ph[MU]);
-  if constexpr (LDSL && CVC_HOP_PIN > 0) pin_regs(acc, 12);
+  if constexpr (LDSL && (CVC_HOP_PIN == 1 || CVC_HOP_PIN == 2)) pin_regs(acc, 12);
   // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
   ld.template proj<MU, true>(h, EO ? sp >> 1 : sp, MU == 3);
-  if constexpr (LDSL && CVC_HOP_PIN > 1) pin_regs(&h[0][0], 6);
+  if constexpr (LDSL && CVC_HOP_PIN == 2) pin_regs(&h[0][0], 6);
   if constexpr (LDSL) link_apply<R, RECON, MU, true>(acc, h, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
   else link_apply<R, RECON, MU, true>(acc, h, U + (size_t)MU * KL * P.V + s, P.V, P.ph[MU]);
   if constexpr (LDSL && CVC_HOP_PIN > 0) pin_regs(acc, 12);

@@ -188,6 +188,16 @@ int cvchip_wi(cvchip_ctx *ctx, double *wi, const double *conn);
 int cvchip_pi_q(cvchip_ctx *ctx, double *piq, const double *conn, const double contact[8],
                 const int src[4]);
 
+/* The scalar vacuum polarisation on qhat^2 orbits (no reference counterpart:
+ * the reference leaves this to analyse_piq.c / make_q2orbits.c, which it does
+ * not ship).  From piq of cvchip_pi_q: Pi(qhat^2) = -sum_mu Re Pi_mu_mu(q) /
+ * (3 qhat^2), qhat_mu = 2 sin(pi n_mu / L_mu), averaged over the momenta of
+ * equal qhat^2 (relative 1e-12); q = 0 is left out.  Orbits in increasing
+ * qhat^2: qhat2[k], pi[k], the standard deviation pi_sd[k] within the orbit and
+ * count[k]; *n_orbits of them (error if more than max_orbits).  Host only. */
+int cvchip_piq_orbits(const int dims[4], const double *piq, int max_orbits, double *qhat2, double *pi,
+                      double *pi_sd, int *count, int *n_orbits);
+
 /* The whole path for one source point: 60 up + 60 down solves (mu and -mu)
  * from point sources at y and y + nu^, then the contraction.  Replaces the
  * propagator reads and contraction of cvc_exact2_xspace.c:401-861 with

@@ -80,6 +80,15 @@ def test_piq_mixed_precision_24x48(chip):
               f"max {err.max():.2e}, median {np.median(err):.2e}; q = 0: {err0:.2e} of max |Pi|", flush=True)
         assert err.max() < 1e-7
         assert err0 < 1e-9
+        # the scalar vacuum polarisation on qhat^2 orbits, qhat^2 <= 4
+        q2, p64, _, cnt = chip.piq_orbits(dims, res[chip.PREC_F64])
+        q2m, pm, _, _ = chip.piq_orbits(dims, res[prec])
+        sel = q2 <= 4.0
+        assert np.array_equal(q2, q2m)
+        eo = np.abs(pm[sel] - p64[sel]) / np.abs(p64[sel])
+        print(f"prec {prec}: Pi(qhat^2) on {int(sel.sum())} orbits qhat^2 <= 4 ({int(cnt[sel].sum())} momenta): "
+              f"max relative error {eo.max():.2e}", flush=True)
+        assert eo.max() < 1e-7
 
 
 @pytest.mark.timeout(900)
