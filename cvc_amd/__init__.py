@@ -23,6 +23,8 @@ ERR_NULL, ERR_NOCONV = -2, -3
 EXPORTS = [
     "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
     "cvchip_ctx_set_solver_flags", "cvchip_piq_orbits",
+    "cvchip_lime_write_contraction", "cvchip_lime_read_contraction", "cvchip_lime_read_gauge",
+    "cvchip_lime_write_gauge",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
@@ -99,6 +101,12 @@ def lib():
         L.cvchip_gauge_random.argtypes = [C.POINTER(C.c_int), C.c_uint64, C.c_double, _dp]
         L.cvchip_piq_orbits.argtypes = [C.POINTER(C.c_int), _dp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]
+        _u32p = C.POINTER(C.c_uint32)
+        L.cvchip_lime_write_contraction.argtypes = [C.c_char_p, _dp, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                                    C.c_char_p, C.c_int, C.c_int, _u32p]
+        L.cvchip_lime_read_contraction.argtypes = [C.c_char_p, _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _u32p]
+        L.cvchip_lime_read_gauge.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, _u32p]
+        L.cvchip_lime_write_gauge.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, _u32p]
         _lib = L
     return _lib
 
@@ -142,6 +150,40 @@ def piq_orbits(dims, piq, max_orbits=1 << 20):
                                    cnt.ctypes.data_as(C.POINTER(C.c_int)), C.byref(no)))
     k = no.value
     return q2[:k], pi[:k], sd[:k], cnt[:k]
+
+
+def lime_write_contraction(filename, s, dims, prec=64, type="cvc - cvc in position space, all 16 components",
+                           nconf=0, source=0):
+    """write_lime_contraction: s float64[V, N, 2] -> LIME file; returns the (suma, sumb) checksum."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_write_contraction(os.fsencode(filename), _ptr(s), int(s.shape[1]), int(prec),
+                                               (C.c_int * 4)(*dims), type.encode(), int(nconf), int(source), ck))
+    return int(ck[0]), int(ck[1])
+
+
+def lime_read_contraction(filename, dims, N=16, position=0):
+    """read_lime_contraction: -> (float64[V, N, 2], (suma, sumb))."""
+    s = np.empty((int(np.prod(dims)), N, 2))
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_read_contraction(os.fsencode(filename), _ptr(s), int(N), int(position),
+                                              (C.c_int * 4)(*dims), ck))
+    return s, (int(ck[0]), int(ck[1]))
+
+
+def lime_read_gauge(filename, dims):
+    """read_lime_gauge_field_doubleprec: ILDG file -> (float64[V, 4, 18], (suma, sumb))."""
+    g = np.empty((int(np.prod(dims)), 4, 18))
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_read_gauge(os.fsencode(filename), (C.c_int * 4)(*dims), _ptr(g), ck))
+    return g, (int(ck[0]), int(ck[1]))
+
+
+def lime_write_gauge(filename, dims, gauge, prec=64):
+    g = np.ascontiguousarray(gauge, dtype=np.float64)
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_write_gauge(os.fsencode(filename), (C.c_int * 4)(*dims), _ptr(g), int(prec), ck))
+    return int(ck[0]), int(ck[1])
 
 
 def point_source(V, site, spin, colour) -> np.ndarray:

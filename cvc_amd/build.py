@@ -80,6 +80,10 @@ def build(verbose: bool = False) -> str:
     shutil.move(tmp, LIB)
     # the drop-in shim is linked by a reference program, not into libcvchip.so
     _run_all([["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-c", DROPIN_SRC, "-o", DROPIN_OBJ]])
+    # the drop-in contraction program (replaces the reference's cvc_exact2_xspace)
+    prog = os.path.join(HERE, "dropin", "cvc_exact2_xspace")
+    _run_all([["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", prog + ".cc", f"-L{HERE}", "-lcvchip",
+               "-Wl,-rpath,$ORIGIN/..", "-o", prog]])
     # a stand-in reference program that links the shim (used by the drop-in test)
     drv = os.path.join(os.path.dirname(HERE), "tests", "dropin")
     if os.path.exists(os.path.join(drv, "driver.cc")):

@@ -206,6 +206,34 @@ int cvchip_cvc_source(cvchip_ctx *ctx, double *conn, double contact[8], const in
                       double sp, int niter_max, int prec, double delta,
                       cvchip_stats *stats);
 
+/* ------------------------------------------------ LIME / SciDAC / ILDG files
+ * Host-only readers and writers of the reference's file formats (no GPU).
+ * checksum, when not NULL, receives the SciDAC checksum (suma, sumb) of the
+ * binary data (reference dml.c:41-53). */
+
+/* write_lime_contraction (contractions_io.c:355-420): records
+ * "cvc_contraction-format" (XML: type, precision, components, lx, ly, lz, lt,
+ * nconf, source, date), "scidac-binary-data" (per site, t x y z order with z
+ * fastest, N complex numbers, big-endian, prec 64 or 32) and
+ * "scidac-checksum".  s: double[V][N][2] (conn layout for N = 16). */
+int cvchip_lime_write_contraction(const char *filename, const double *s, int N, int prec, const int dims[4],
+                                  const char *type, int nconf, int source, uint32_t checksum[2]);
+/* read_lime_contraction (contractions_io.c:430-505): the position-th
+ * scidac-binary-data record (0-based), precision from its length; an error if
+ * a following scidac-checksum record disagrees with the data. */
+int cvchip_lime_read_contraction(const char *filename, double *s, int N, int position, const int dims[4],
+                                 uint32_t checksum[2]);
+/* read_lime_gauge_field_doubleprec / _singleprec (io.c:202-347, 350-481): the
+ * first ildg-binary-data record (sites t z y x with x fastest, links x y z t,
+ * big-endian, 64 or 32 bit) into gauge[V][4][18] (the reference layout);
+ * an error if a scidac-checksum record disagrees. */
+int cvchip_lime_read_gauge(const char *filename, const int dims[4], double *gauge, uint32_t checksum[2]);
+/* An ILDG file of gauge[V][4][18] that the reader above and the reference's
+ * read_lime_gauge_field_doubleprec accept ("ildg-format", "ildg-binary-data",
+ * "scidac-checksum").  No reference counterpart (the path reads gauges only). */
+int cvchip_lime_write_gauge(const char *filename, const int dims[4], const double *gauge, int prec,
+                            uint32_t checksum[2]);
+
 /* Gradient of L = sum_mu sum_{q in Q} Re Pi_{mu mu}(q) (Pi(q) as in cvchip_pi_q,
  * Q = {q : |q| <= qmax}, q_mu = 2 pi n_mu / L_mu with n_mu in (-L_mu/2, L_mu/2])
  * with respect to the twisted mass mu and every gauge link, by adjoint solves:

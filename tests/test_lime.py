@@ -1,0 +1,121 @@
+"""LIME / SciDAC / ILDG file I/O (CPU).
+
+An independent Python reader of the LIME record layout (144-byte big-endian
+header: magic 0x456789ab, version, MB/ME flags, length, type; payload padded to
+8 bytes) and of the SciDAC checksum (zlib crc32 of each site's stored bytes,
+rotated by rank mod 29 / 31 and XOR-ed; reference dml.c:41-53) checks what the
+library writes; files written here by Python in the ILDG layout of the
+reference's reader (io.c:202-347) are read back by the library."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+import cvc_amd
+
+
+def records(path):
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        magic, ver, flags, n = struct.unpack(">IHHQ", data[pos:pos + 16])
+        assert magic == 0x456789AB and ver == 1
+        typ = data[pos + 16:pos + 144].split(b"\0")[0].decode()
+        payload = data[pos + 144:pos + 144 + n]
+        pad = (8 - n % 8) % 8
+        assert data[pos + 144 + n:pos + 144 + n + pad] == b"\0" * pad
+        out.append((typ, bool(flags & 0x8000), bool(flags & 0x4000), payload))
+        pos += 144 + n + pad
+    assert pos == len(data)
+    return out
+
+
+def scidac(payload, site_bytes, ranks):
+    a = b = 0
+    for k, r in enumerate(ranks):
+        w = zlib.crc32(payload[k * site_bytes:(k + 1) * site_bytes]) & 0xFFFFFFFF
+        ra, rb = r % 29, r % 31
+        a ^= ((w << ra) | (w >> (32 - ra))) & 0xFFFFFFFF if ra else w
+        b ^= ((w << rb) | (w >> (32 - rb))) & 0xFFFFFFFF if rb else w
+    return a, b
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_contraction_file(tmp_path, prec):
+    dims = (4, 2, 3, 2)
+    V = int(np.prod(dims))
+    rng = np.random.default_rng(prec)
+    s = rng.standard_normal((V, 16, 2))
+    path = tmp_path / "cvc2_v_x.0007"
+    ck = cvc_amd.lime_write_contraction(path, s, dims, prec=prec, nconf=7, source=0)
+    recs = records(path)
+    assert [r[0] for r in recs] == ["cvc_contraction-format", "scidac-binary-data", "scidac-checksum"]
+    assert [(r[1], r[2]) for r in recs] == [(True, True), (True, False), (False, True)]
+    xml = recs[0][3].decode()
+    for tag, val in (("precision", prec), ("components", 16), ("lx", 2), ("ly", 3), ("lz", 2), ("lt", 4),
+                     ("nconf", 7), ("source", 0)):
+        assert f"<{tag}>{val}</{tag}>" in xml
+    fmt = ">f8" if prec == 64 else ">f4"
+    raw = np.frombuffer(recs[1][3], dtype=fmt).reshape(V, 16, 2)
+    want = s if prec == 64 else s.astype(np.float32).astype(np.float64)
+    assert np.array_equal(raw.astype(np.float64), want)
+    site = 16 * 2 * (8 if prec == 64 else 4)
+    assert scidac(recs[1][3], site, range(V)) == ck
+    assert f"<suma>{ck[0]:08x}</suma>" in recs[2][3].decode() and f"<sumb>{ck[1]:08x}</sumb>" in recs[2][3].decode()
+    back, ck2 = cvc_amd.lime_read_contraction(path, dims)
+    assert ck2 == ck and np.array_equal(back, want)
+
+
+def test_contraction_checksum_mismatch(tmp_path):
+    dims = (2, 2, 2, 2)
+    s = np.ones((16, 16, 2))
+    path = tmp_path / "c"
+    cvc_amd.lime_write_contraction(path, s, dims)
+    raw = bytearray(path.read_bytes())
+    hdr = raw.find(b"scidac-binary-data") - 16
+    raw[hdr + 144 + 3] ^= 0x40                     # flip a bit of the first value
+    path.write_bytes(bytes(raw))
+    with pytest.raises(cvc_amd.CvchipError):
+        cvc_amd.lime_read_contraction(path, dims)
+    with pytest.raises(cvc_amd.CvchipError):
+        cvc_amd.lime_read_contraction(path, (2, 2, 2, 4))   # wrong volume
+
+
+def write_ildg_python(path, dims, g, prec=64):
+    """An ILDG file in the layout read by io.c:202-347, written without the library."""
+    T, LX, LY, LZ = dims
+    G = g.reshape(T, LX, LY, LZ, 4, 18)
+    site = np.concatenate([G[..., 1, :], G[..., 2, :], G[..., 3, :], G[..., 0, :]], -1)   # x, y, z, t
+    site = site.transpose(0, 3, 2, 1, 4)                                                  # t, z, y, x
+    payload = site.astype(">f8" if prec == 64 else ">f4").tobytes()
+    a, b = scidac(payload, 72 * (prec // 8), range(T * LX * LY * LZ))
+    xml = b"<?xml version=\"1.0\"?><ildgFormat><precision>%d</precision></ildgFormat>" % prec
+    ck = (b"<?xml version=\"1.0\" encoding=\"UTF-8\"?><scidacChecksum><version>1.0</version>"
+          b"<suma>%08x</suma><sumb>%08x</sumb></scidacChecksum>" % (a, b))
+    with open(path, "wb") as f:
+        for typ, flags, data in (("ildg-format", 0x8000, xml), ("ildg-binary-data", 0, payload),
+                                 ("scidac-checksum", 0x4000, ck)):
+            f.write(struct.pack(">IHHQ", 0x456789AB, 1, flags, len(data)) + typ.encode().ljust(128, b"\0"))
+            f.write(data + b"\0" * ((8 - len(data) % 8) % 8))
+    return a, b
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_gauge_file(tmp_path, prec):
+    dims = (4, 2, 3, 2)
+    g = cvc_amd.gauge_random(dims, 3, 0.3)
+    path = tmp_path / "conf.0003"
+    ck = write_ildg_python(path, dims, g, prec)
+    back, ck2 = cvc_amd.lime_read_gauge(path, dims)
+    want = g if prec == 64 else g.astype(np.float32).astype(np.float64)
+    assert ck2 == ck and np.array_equal(back, want)
+    # the library's writer produces a file the same reader returns unchanged, same checksum
+    path2 = tmp_path / "conf2"
+    ck3 = cvc_amd.lime_write_gauge(path2, dims, g, prec)
+    assert ck3 == ck
+    back2, _ = cvc_amd.lime_read_gauge(path2, dims)
+    assert np.array_equal(back2, want)
+    assert [r[0] for r in records(path2)] == ["ildg-format", "ildg-binary-data", "scidac-checksum"]
