@@ -24,7 +24,8 @@ EXPORTS = [
     "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
     "cvchip_ctx_set_solver_flags", "cvchip_piq_orbits",
     "cvchip_lime_write_contraction", "cvchip_lime_read_contraction", "cvchip_lime_read_gauge",
-    "cvchip_lime_write_gauge",
+    "cvchip_lime_write_gauge", "cvchip_gamma_tables", "cvchip_contract_twopoint",
+    "cvchip_field_contract_twopoint",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
@@ -107,6 +108,11 @@ def lib():
         L.cvchip_lime_read_contraction.argtypes = [C.c_char_p, _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _u32p]
         L.cvchip_lime_read_gauge.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, _u32p]
         L.cvchip_lime_write_gauge.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, _u32p]
+        L.cvchip_gamma_tables.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cvchip_contract_twopoint.argtypes = [_vp, _dp, C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int,
+                                               C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cvchip_field_contract_twopoint.argtypes = [_vp, _dp, C.c_int, C.c_int, _vp, _vp, C.c_int,
+                                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -184,6 +190,14 @@ def lime_write_gauge(filename, dims, gauge, prec=64):
     ck = (C.c_uint32 * 2)()
     _check(lib().cvchip_lime_write_gauge(os.fsencode(filename), (C.c_int * 4)(*dims), _ptr(g), int(prec), ck))
     return int(ck[0]), int(ck[1])
+
+
+def gamma_tables():
+    """gamma_permutation, gamma_sign: int[16][24] each (the library's own tables)."""
+    perm = np.zeros((16, 24), dtype=np.int32)
+    sign = np.zeros((16, 24), dtype=np.int32)
+    _check(lib().cvchip_gamma_tables(perm.ctypes.data_as(C.POINTER(C.c_int)), sign.ctypes.data_as(C.POINTER(C.c_int))))
+    return perm, sign
 
 
 def point_source(V, site, spin, colour) -> np.ndarray:
@@ -354,6 +368,26 @@ class Context:
         return rc, st.as_dict()
 
     # -- observables -----------------------------------------------------------
+    def contract_twopoint(self, idsource, idsink, chi, phi, n_c=3, snk_mom=None, src=(0, 0, 0, 0)):
+        """Meson two-point function C[T] (complex) of host propagator columns (4 n_c arrays [V,24])."""
+        ca = [np.ascontiguousarray(a, dtype=np.float64) for a in chi]
+        pa = [np.ascontiguousarray(a, dtype=np.float64) for a in phi]
+        out = np.zeros((self.dims[0], 2))
+        _check(lib().cvchip_contract_twopoint(self._h, _ptr(out), int(idsource), int(idsink),
+                                              (_dp * len(ca))(*[_ptr(a) for a in ca]),
+                                              (_dp * len(pa))(*[_ptr(a) for a in pa]), int(n_c),
+                                              (C.c_int * 3)(*snk_mom) if snk_mom is not None else None,
+                                              (C.c_int * 4)(*src)))
+        return out[:, 0] + 1j * out[:, 1]
+
+    def field_contract_twopoint(self, idsource, idsink, chi: Field, phi: Field, n_c=3, snk_mom=None,
+                                src=(0, 0, 0, 0)):
+        out = np.zeros((self.dims[0], 2))
+        _check(lib().cvchip_field_contract_twopoint(self._h, _ptr(out), int(idsource), int(idsink), chi._h, phi._h,
+                                                    int(n_c), (C.c_int * 3)(*snk_mom) if snk_mom is not None else None,
+                                                    (C.c_int * 4)(*src)))
+        return out[:, 0] + 1j * out[:, 1]
+
     def cvc_contract(self, props, src):
         assert len(props) == 120
         arrs = [np.ascontiguousarray(p, dtype=np.float64) for p in props]

@@ -206,6 +206,26 @@ int cvchip_cvc_source(cvchip_ctx *ctx, double *conn, double contact[8], const in
                       double sp, int niter_max, int prec, double delta,
                       cvchip_stats *stats);
 
+/* gamma_permutation / gamma_sign (global.h:143, filled by init_gamma,
+ * cvc_utils.c:707-1520): the 16 gamma matrices as signed permutations of the
+ * 24 reals of a spinor, index 0-3 gamma_mu, 4 identity, 5 gamma5, 6-9
+ * gamma_mu gamma5, 10-15 gamma_0gamma_1, _0_2, _0_3, _1_2, _1_3, _2_3. */
+int cvchip_gamma_tables(int perm[16][24], int sign[16][24]);
+
+/* contract_twopoint / contract_twopoint_snk_momentum (cvc_utils.c:3075-3235):
+ * contr[T][2] += sum_x e^{i p.(x - y)} sum_{mu, c} s_mu chi[p(mu) n_c + c]^dagger
+ * gamma5 Gamma_idsink phi[mu n_c + c] per time slice (times -i for an
+ * imaginary source gamma); chi, phi: 4 n_c spinor fields double[V][24];
+ * snk_mom NULL for zero momentum, else 2 pi snk_mom / L; src: the source
+ * point, whose spatial part enters the phase (g_source_location). */
+int cvchip_contract_twopoint(cvchip_ctx *ctx, double *contr, int idsource, int idsink,
+                             const double *const *chi, const double *const *phi, int n_c,
+                             const int snk_mom[3], const int src[4]);
+/* The same on device-resident FP64 sets of 4 n_c columns (column mu n_c + c). */
+int cvchip_field_contract_twopoint(cvchip_ctx *ctx, double *contr, int idsource, int idsink,
+                                   const cvchip_field *chi, const cvchip_field *phi, int n_c,
+                                   const int snk_mom[3], const int src[4]);
+
 /* ------------------------------------------------ LIME / SciDAC / ILDG files
  * Host-only readers and writers of the reference's file formats (no GPU).
  * checksum, when not NULL, receives the SciDAC checksum (suma, sumb) of the

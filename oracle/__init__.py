@@ -58,6 +58,8 @@ def lib():
         L.orc_invert_Qtm_her.argtypes = [P, dp, dp, dp, C.c_double, C.c_int, dp]
         L.orc_cvc_contract.argtypes = [P, dp, C.POINTER(dp), C.POINTER(C.c_int), dp, dp]
         L.orc_wi.argtypes = [P, dp, dp]
+        L.orc_contract_twopoint.argtypes = [P, dp, C.c_int, C.c_int, C.POINTER(dp), C.POINTER(dp), C.c_int,
+                                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_plaquette.argtypes = [P, dp]
         L.orc_plaquette.restype = C.c_double
         L.orc_gamma_tables.argtypes = [C.POINTER((C.c_int * 24) * 16), C.POINTER((C.c_int * 24) * 16)]
@@ -162,6 +164,19 @@ class Lattice:
         out = np.empty((self.V, 4, 2))
         lib().orc_wi(C.byref(self._s), _dp(np.ascontiguousarray(conn)), _dp(out))
         return out
+
+    def contract_twopoint(self, idsource, idsink, chi, phi, n_c=3, snk_mom=None, src=(0, 0, 0, 0)):
+        """contract_twopoint(_snk_momentum) (cvc_utils.c:3075-3235): complex C[T]."""
+        assert len(chi) == 4 * n_c and len(phi) == 4 * n_c
+        ca = [np.ascontiguousarray(a, dtype=np.float64) for a in chi]
+        pa = [np.ascontiguousarray(a, dtype=np.float64) for a in phi]
+        cp = (C.POINTER(C.c_double) * len(ca))(*[_dp(a) for a in ca])
+        pp = (C.POINTER(C.c_double) * len(pa))(*[_dp(a) for a in pa])
+        out = np.zeros((self.dims[0], 2))
+        mom = (C.c_int * 3)(*snk_mom) if snk_mom is not None else None
+        lib().orc_contract_twopoint(C.byref(self._s), _dp(out), int(idsource), int(idsink), cp, pp, int(n_c), mom,
+                                    (C.c_int * 4)(*src))
+        return out[:, 0] + 1j * out[:, 1]
 
     def plaquette(self, gauge):
         return lib().orc_plaquette(C.byref(self._s), _dp(gauge))

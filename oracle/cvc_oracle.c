@@ -549,3 +549,60 @@ double orc_plaquette(const orc_lat *L, const double *gauge) {
       }
   return pl / ((double)L->V * 18.);
 }
+
+/* ------------------------------------------------- meson two-point function */
+
+void orc_contract_twopoint(const orc_lat *L, double *contr, int idsource, int idsink,
+                           const double *const *chi, const double *const *phi, int n_c,
+                           const int *snk_mom, const int src[4]) {
+  /* cvc_utils.c:3084-3096: source column map, imaginary flag and signs */
+  int psource[4];
+  double ssource[4];
+  for (int m = 0; m < 4; m++) {
+    psource[m] = L->perm[idsource][6 * m] / 6;
+    ssource[m] = L->sign[idsource][6 * m] * L->sign[5][L->perm[idsource][6 * m]];
+  }
+  const int isimag = L->perm[idsource][0] % 2;
+  double px = 0., py = 0., pz = 0.;
+  if (snk_mom) {
+    px = 2. * ORC_PI * snk_mom[0] / L->LX;
+    py = 2. * ORC_PI * snk_mom[1] / L->LY;
+    pz = 2. * ORC_PI * snk_mom[2] / L->LZ;
+  }
+  double spinor1[24], spinor2[24];
+  for (int x0 = 0; x0 < L->T; x0++) {
+    for (int x1 = 0; x1 < L->LX; x1++)
+    for (int x2 = 0; x2 < L->LY; x2++)
+    for (int x3 = 0; x3 < L->LZ; x3++) {
+      const long iix = lex(L, x0, x1, x2, x3);
+      const double phase = (x1 - src[1]) * px + (x2 - src[2]) * py + (x3 - src[3]) * pz;
+      const double cphase = cos(phase), sphase = sin(phase);
+      double tmp[2] = {0., 0.};
+      for (int mu = 0; mu < 4; mu++)
+        for (int c = 0; c < n_c; c++) {
+          const double *ph = phi[mu * n_c + c] + 24 * iix;
+          const double *ch = chi[psource[mu] * n_c + c] + 24 * iix;
+          /* spinor1 = Gamma_sink phi, spinor2 = gamma5 spinor1 (cvc_linalg.h:1344-1368) */
+          for (int k = 0; k < 24; k++) spinor1[k] = ph[L->perm[idsink][k]] * L->sign[idsink][k];
+          for (int k = 0; k < 24; k++) spinor2[k] = spinor1[L->perm[5][k]] * L->sign[5][k];
+          /* w = chi^dagger spinor2 (cvc_linalg.h:669) */
+          double wr = 0., wi = 0.;
+          for (int k = 0; k < 12; k++) {
+            wr += ch[2 * k] * spinor2[2 * k] + ch[2 * k + 1] * spinor2[2 * k + 1];
+            wi += ch[2 * k] * spinor2[2 * k + 1] - ch[2 * k + 1] * spinor2[2 * k];
+          }
+          tmp[0] += ssource[mu] * wr;
+          tmp[1] += ssource[mu] * wi;
+        }
+      const double re = tmp[0] * cphase - tmp[1] * sphase;
+      const double im = tmp[0] * sphase + tmp[1] * cphase;
+      if (!isimag) {
+        contr[2 * x0] += re;
+        contr[2 * x0 + 1] += im;
+      } else {
+        contr[2 * x0] += im;
+        contr[2 * x0 + 1] += -re;
+      }
+    }
+  }
+}

@@ -85,6 +85,16 @@ void orc_wi(const orc_lat *L, const double *conn, double *wi);
 /* Average plaquette, cvc_utils.c:384-405. */
 double orc_plaquette(const orc_lat *L, const double *gauge);
 
+/* Meson two-point function (cvc_utils.c:3075-3140 contract_twopoint; with a
+ * sink momentum cvc_utils.c:3143-3235 contract_twopoint_snk_momentum, serial
+ * variant): contr[T][2] += sum_x e^{i p.(x - y)} sum_{mu, c} ssource[mu]
+ * chi[psource[mu] n_c + c](x)^dagger gamma5 Gamma_sink phi[mu n_c + c](x),
+ * times -i for an imaginary source gamma; snk_mom NULL for p = 0, src the
+ * source point (its spatial part enters the phase). */
+void orc_contract_twopoint(const orc_lat *L, double *contr, int idsource, int idsink,
+                           const double *const *chi, const double *const *phi, int n_c,
+                           const int *snk_mom, const int src[4]);
+
 #ifdef __cplusplus
 }
 #endif

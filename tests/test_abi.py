@@ -37,3 +37,13 @@ def test_errors_without_gpu_work(chip):
     import pytest
     with pytest.raises(chip.CvchipError):
         chip.gauge_random((0, 4, 4, 4), 1, 0.3)
+
+
+def test_gamma_tables_equal_the_oracle(orc):
+    """The library builds its own gamma tables (from the chiral gamma_mu); they equal
+    the oracle's, which are pinned to the reference's documented matrices."""
+    import numpy as np
+    import cvc_amd
+    perm, sign = cvc_amd.gamma_tables()
+    po, so = orc.gamma_tables()
+    assert np.array_equal(perm, np.asarray(po)) and np.array_equal(sign, np.asarray(so))
