@@ -24,7 +24,8 @@ EXPORTS = [
     "cvchip_ctx_create", "cvchip_ctx_destroy", "cvchip_ctx_set_params", "cvchip_ctx_set_link_recon",
     "cvchip_ctx_set_solver_flags", "cvchip_piq_orbits",
     "cvchip_lime_write_contraction", "cvchip_lime_read_contraction", "cvchip_lime_read_gauge",
-    "cvchip_lime_write_gauge", "cvchip_gamma_tables", "cvchip_contract_twopoint",
+    "cvchip_lime_write_gauge", "cvchip_lime_read_spinor", "cvchip_lime_write_spinor",
+    "cvchip_lime_write_propagator", "cvchip_cvc_propagators", "cvchip_gamma_tables", "cvchip_contract_twopoint",
     "cvchip_field_contract_twopoint",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
@@ -108,6 +109,11 @@ def lib():
         L.cvchip_lime_read_contraction.argtypes = [C.c_char_p, _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _u32p]
         L.cvchip_lime_read_gauge.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, _u32p]
         L.cvchip_lime_write_gauge.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, _u32p]
+        L.cvchip_lime_read_spinor.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_int, C.c_double, _dp, _u32p]
+        L.cvchip_lime_write_spinor.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, C.c_int, _u32p]
+        L.cvchip_lime_write_propagator.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, _u32p]
+        L.cvchip_cvc_propagators.argtypes = [_vp, C.POINTER(_dp), C.POINTER(C.c_int), C.c_double, C.c_int, C.c_int,
+                                             C.c_double, C.POINTER(Stats)]
         L.cvchip_gamma_tables.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cvchip_contract_twopoint.argtypes = [_vp, _dp, C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int,
                                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -189,6 +195,33 @@ def lime_write_gauge(filename, dims, gauge, prec=64):
     g = np.ascontiguousarray(gauge, dtype=np.float64)
     ck = (C.c_uint32 * 2)()
     _check(lib().cvchip_lime_write_gauge(os.fsencode(filename), (C.c_int * 4)(*dims), _ptr(g), int(prec), ck))
+    return int(ck[0]), int(ck[1])
+
+
+def lime_read_spinor(filename, dims, position=0, kappa=0.0):
+    """read_lime_spinor: record `position` (0 up, 1 down) -> (float64[V, 24], (suma, sumb));
+    a file without such a record is read as CMI (scaled by 2 kappa)."""
+    s = np.empty((int(np.prod(dims)), 24))
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_read_spinor(os.fsencode(filename), (C.c_int * 4)(*dims), int(position), float(kappa),
+                                         _ptr(s), ck))
+    return s, (int(ck[0]), int(ck[1]))
+
+
+def lime_write_spinor(filename, dims, s, append=False, prec=64):
+    """write_lime_spinor: s float64[V, 24] -> binary-data + checksum records; returns (suma, sumb)."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_write_spinor(os.fsencode(filename), (C.c_int * 4)(*dims), _ptr(s), int(bool(append)),
+                                          int(prec), ck))
+    return int(ck[0]), int(ck[1])
+
+
+def lime_write_propagator(filename, dims, s, prec=64):
+    """write_propagator: new file, etmc-propagator-format record, then write_lime_spinor."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    ck = (C.c_uint32 * 2)()
+    _check(lib().cvchip_lime_write_propagator(os.fsencode(filename), (C.c_int * 4)(*dims), _ptr(s), int(prec), ck))
     return int(ck[0]), int(ck[1])
 
 
@@ -431,6 +464,15 @@ class Context:
         if force:
             Fc = (F[..., 0::2] + 1j * F[..., 1::2]).reshape(self.V, 4, 3, 3)
         return float(Lv[0]), float(dmu[0]), Fc, st.as_dict()
+
+    def cvc_propagators(self, src, sp, niter_max=1000, prec=PREC_F64, delta=0.1):
+        """The 120 propagators of one source point, float64[120, V, 24] in g_spinor_field order."""
+        props = np.empty((120, self.V, 24))
+        ptrs = (_dp * 120)(*[_ptr(props[k]) for k in range(120)])
+        st = Stats()
+        _check(lib().cvchip_cvc_propagators(self._h, ptrs, (C.c_int * 4)(*src), float(sp), int(niter_max), int(prec),
+                                            float(delta), C.byref(st)))
+        return props, st.as_dict()
 
     def cvc_source(self, src, sp, niter_max=1000, prec=PREC_F64, delta=0.1):
         conn = np.empty((self.V, 16, 2))

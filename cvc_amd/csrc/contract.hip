@@ -555,3 +555,28 @@ extern "C" int cvchip_cvc_source(cvchip_ctx *c, double *conn, double contact[8],
   field_free(dn);
   return rc;
 }
+
+// The propagators of cvchip_cvc_source, downloaded into the reference layout.
+extern "C" int cvchip_cvc_propagators(cvchip_ctx *c, double *const *props, const int src[4], double sp, int niter_max,
+                                      int prec, double delta, cvchip_stats *stats) {
+  if (!c || !props || !src) {
+    set_error("NULL argument to cvchip_cvc_propagators");
+    return CVCHIP_ERR_NULL;
+  }
+  for (int k = 0; k < 120; k++)
+    if (!props[k]) {
+      set_error("propagator %d is NULL", k);
+      return CVCHIP_ERR_NULL;
+    }
+  Field up, dn;
+  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+  cvchip_stats tot;
+  memset(&tot, 0, sizeof(tot));
+  if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
+  for (int k = 0; k < 120 && !rc; k++) rc = download_aos(c, k < 60 ? up : dn, props[k], k % 60, 1);
+  if (stats) *stats = tot;
+  field_free(up);
+  field_free(dn);
+  return rc;
+}

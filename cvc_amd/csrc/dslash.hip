@@ -475,16 +475,23 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
   const uint32_t S = (uint32_t)(block_for(N) / N);
   int want = (c->dslash_policy >> 4) & 15;
   uint32_t yb = (uint32_t)((c->dslash_policy >> 8) & 255);
-  // automatic: bricks of 4 slices x 4 x-planes x 8 y-rows for the multi-rhs
+  // automatic: bricks of 4 slices x 4 x-planes x yb y-rows for the multi-rhs
   // operator, else one slice per XCD in y blocks (both measured at 64x32^3
   // with 12 rhs: 3.98 and 4.19 ms against 4.75 ms for XCD chunks of a
-  // slice); XCD chunks of a slice for a single rhs (0.48 ms against 0.54)
+  // slice); XCD chunks of a slice for a single rhs (0.48 ms against 0.54).
+  // The brick's y extent shrinks as N grows so that yb * N stays near 96
+  // (measured at 64x32^3: 12 rhs best at yb 8, 24 rhs at 4, 48 rhs at 2).
   const bool autom = want == 0;
   if (autom) want = N >= 8 ? 4 : 1;
   if (yb == 0) {
     yb = 1;
-    for (uint32_t y : {autom && want == 4 ? 8u : 16u, 8u, 4u, 2u})
-      if (L.LY % (int)y == 0) { yb = y; break; }
+    if (autom && want == 4) {
+      for (uint32_t y : {8u, 4u, 2u})
+        if (y * (uint32_t)N <= 96u && L.LY % (int)y == 0) { yb = y; break; }
+    } else {
+      for (uint32_t y : {16u, 8u, 4u, 2u})
+        if (L.LY % (int)y == 0) { yb = y; break; }
+    }
   }
   Traversal t{};
   uint32_t bt = (uint32_t)((c->dslash_policy >> 16) & 15), bx = (uint32_t)((c->dslash_policy >> 20) & 15);

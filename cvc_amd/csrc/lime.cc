@@ -460,3 +460,219 @@ extern "C" int cvchip_lime_write_gauge(const char *filename, const int dims[4], 
   }
   return CVCHIP_OK;
 }
+
+// ------------------------------------------------------------ spinors (propagators)
+
+namespace {
+
+// a spinor field of the reference layout s[24 ix + 2 (3 spin + colour) + re/im]
+// to / from the file's sites in t, z, y, x order (x fastest), 24 big-endian
+// reals per site (read_binary_spinor_data / write_binary_spinor_data,
+// reference propagator_io.c:141-200 and 344-405); rank = file position
+void spinor_to_file(const double *s, const int d[4], int w, std::vector<unsigned char> &data, Checksum &ck) {
+  const int T = d[0], LX = d[1], LY = d[2], LZ = d[3];
+  data.resize((size_t)T * LX * LY * LZ * 24 * w);
+  uint64_t rank = 0;
+  for (int t = 0; t < T; t++)
+    for (int z = 0; z < LZ; z++)
+      for (int y = 0; y < LY; y++)
+        for (int x = 0; x < LX; x++, rank++) {
+          unsigned char *p = data.data() + rank * 24 * w;
+          const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
+          for (int e = 0; e < 24; e++) {
+            if (w == 8) put_double_be(p + e * 8, s[ix * 24 + e]);
+            else put_float_be(p + e * 4, (float)s[ix * 24 + e]);
+          }
+          ck.accum(rank, p, (size_t)24 * w);
+        }
+}
+
+void spinor_from_file(double *s, const int d[4], int w, const std::vector<unsigned char> &data, Checksum &ck) {
+  const int T = d[0], LX = d[1], LY = d[2], LZ = d[3];
+  uint64_t rank = 0;
+  for (int t = 0; t < T; t++)
+    for (int z = 0; z < LZ; z++)
+      for (int y = 0; y < LY; y++)
+        for (int x = 0; x < LX; x++, rank++) {
+          const unsigned char *p = data.data() + rank * 24 * w;
+          ck.accum(rank, p, (size_t)24 * w);
+          const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
+          for (int e = 0; e < 24; e++) s[ix * 24 + e] = w == 8 ? get_double_be(p + e * 8) : (double)get_float_be(p + e * 4);
+        }
+}
+
+// read_cmi (reference propagator_io.c:842-892): raw host-order floats, sites in
+// x, y, z, t order (t fastest), 24 per site, scaled by 2 kappa
+int read_cmi(FILE *f, const char *filename, const int d[4], double kappa, double *s) {
+  const int T = d[0], LX = d[1], LY = d[2], LZ = d[3];
+  const uint64_t V = (uint64_t)T * LX * LY * LZ;
+  if (fseek(f, 0, SEEK_END) != 0 || (uint64_t)ftell(f) != V * 24 * sizeof(float)) {
+    set_error("%s is neither a LIME file with a scidac-binary-data record nor a CMI file of this volume", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  std::vector<float> buf(V * 24);
+  if (fseek(f, 0, SEEK_SET) != 0 || fread(buf.data(), sizeof(float), buf.size(), f) != buf.size()) {
+    set_error("read error on %s", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  const double k2 = 2.0 * kappa;
+  uint64_t pos = 0;
+  for (int x = 0; x < LX; x++)
+    for (int y = 0; y < LY; y++)
+      for (int z = 0; z < LZ; z++)
+        for (int t = 0; t < T; t++, pos++) {
+          const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
+          for (int e = 0; e < 24; e++) s[ix * 24 + e] = k2 * (double)buf[pos * 24 + e];
+        }
+  return CVCHIP_OK;
+}
+
+// write_checksum (reference propagator_io.c:410-466): its own message, XML on one line
+std::string spinor_checksum_xml(const Checksum &c) {
+  char m[512];
+  snprintf(m, sizeof(m),
+           "<?xml version=\"1.0\" encoding=\"UTF-8\"?><scidacChecksum><version>1.0</version><suma>%#010x</suma>"
+           "<sumb>%#010x</sumb></scidacChecksum>",
+           c.a, c.b);
+  return m;
+}
+
+}  // namespace
+
+// read_lime_spinor (reference propagator_io.c:771-840): the position-th
+// scidac-binary-data record (0: up, 1: down of a two-flavour file), 64 or 32
+// bit by its length; a file without such a record is read as CMI (kappa is
+// used only there).  The data checksum is returned and compared with a
+// scidac-checksum record that follows the data record.
+extern "C" int cvchip_lime_read_spinor(const char *filename, const int dims[4], int position, double kappa, double *s,
+                                       uint32_t checksum[2]) {
+  if (!filename || !dims || !s) {
+    set_error("NULL argument to cvchip_lime_read_spinor");
+    return CVCHIP_ERR_NULL;
+  }
+  if (!check_dims(dims) || position < 0) {
+    set_error("invalid spinor read request (position %d)", position);
+    return CVCHIP_ERR_ARG;
+  }
+  FILE *f = fopen(filename, "rb");
+  if (!f) {
+    set_error("could not open %s", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  std::vector<Record> recs;
+  size_t k = 0;
+  const bool lime = scan(f, recs);
+  if (lime) {
+    int got = -1;
+    for (; k < recs.size(); k++)
+      if (recs[k].type == "scidac-binary-data" && ++got == position) break;
+  }
+  if (!lime || k == recs.size()) {
+    const int rc = read_cmi(f, filename, dims, kappa, s);
+    fclose(f);
+    if (rc == CVCHIP_OK && checksum) checksum[0] = checksum[1] = 0;
+    return rc;
+  }
+  const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  int w;
+  if (recs[k].bytes == V * 24 * 8) w = 8;
+  else if (recs[k].bytes == V * 24 * 4) w = 4;
+  else {
+    fclose(f);
+    set_error("wrong length %llu of the spinor record in %s (expected %llu)", (unsigned long long)recs[k].bytes,
+              filename, (unsigned long long)(V * 24 * 8));
+    return CVCHIP_ERR_ARG;
+  }
+  std::vector<unsigned char> data;
+  if (!read_payload(f, recs[k], data)) {
+    fclose(f);
+    set_error("read error on %s", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  Checksum ck;
+  spinor_from_file(s, dims, w, data, ck);
+  int rc = CVCHIP_OK;
+  if (k + 1 < recs.size() && recs[k + 1].type == "scidac-checksum") {
+    std::vector<unsigned char> xml;
+    uint32_t a, b;
+    if (read_payload(f, recs[k + 1], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
+      set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
+      rc = CVCHIP_ERR_ARG;
+    }
+  }
+  fclose(f);
+  if (checksum) {
+    checksum[0] = ck.a;
+    checksum[1] = ck.b;
+  }
+  return rc;
+}
+
+// write_lime_spinor (reference propagator_io.c:582-647): a "scidac-binary-data"
+// record (64 or 32 bit) and a "scidac-checksum" record, appended to the file
+// or in a new one.
+extern "C" int cvchip_lime_write_spinor(const char *filename, const int dims[4], const double *s, int append, int prec,
+                                        uint32_t checksum[2]) {
+  if (!filename || !dims || !s) {
+    set_error("NULL argument to cvchip_lime_write_spinor");
+    return CVCHIP_ERR_NULL;
+  }
+  if (!check_dims(dims) || (prec != 64 && prec != 32)) {
+    set_error("invalid spinor format (precision %d)", prec);
+    return CVCHIP_ERR_ARG;
+  }
+  std::vector<unsigned char> data;
+  Checksum ck;
+  spinor_to_file(s, dims, prec == 64 ? 8 : 4, data, ck);
+  FILE *f = fopen(filename, append ? "ab" : "wb");
+  if (!f) {
+    set_error("could not open %s for writing", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  bool ok = write_record(f, "scidac-binary-data", false, true, data.data(), data.size());
+  const std::string x = spinor_checksum_xml(ck);
+  ok = ok && write_record(f, "scidac-checksum", true, false, x.data(), x.size());
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) {
+    set_error("write error on %s", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  if (checksum) {
+    checksum[0] = ck.a;
+    checksum[1] = ck.b;
+  }
+  return CVCHIP_OK;
+}
+
+// write_propagator (reference propagator_io.c:125-133, the LIME build): a new
+// file with the "etmc-propagator-format" record (write_propagator_format,
+// propagator_io.c:522-576, one flavour), then the spinor as write_lime_spinor.
+// A second flavour is appended with cvchip_lime_write_spinor(append = 1).
+extern "C" int cvchip_lime_write_propagator(const char *filename, const int dims[4], const double *s, int prec,
+                                            uint32_t checksum[2]) {
+  if (!filename || !dims || !s) {
+    set_error("NULL argument to cvchip_lime_write_propagator");
+    return CVCHIP_ERR_NULL;
+  }
+  if (!check_dims(dims) || (prec != 64 && prec != 32)) {
+    set_error("invalid spinor format (precision %d)", prec);
+    return CVCHIP_ERR_ARG;
+  }
+  FILE *f = fopen(filename, "wb");
+  if (!f) {
+    set_error("could not open %s for writing", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  char msg[512];
+  snprintf(msg, sizeof(msg),
+           "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<etmcFormat>\n<field>diracFermion</field>\n<precision>%d"
+           "</precision>\n<flavours>%d</flavours>\n<lx>%d</lx>\n<ly>%d</ly>\n<lz>%d</lz>\n<lt>%d</lt>\n</etmcFormat>",
+           prec, 1, dims[1], dims[2], dims[3], dims[0]);
+  bool ok = write_record(f, "etmc-propagator-format", true, false, msg, strlen(msg));
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) {
+    set_error("write error on %s", filename);
+    return CVCHIP_ERR_ARG;
+  }
+  return cvchip_lime_write_spinor(filename, dims, s, 1, prec, checksum);
+}

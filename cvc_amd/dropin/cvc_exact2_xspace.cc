@@ -1,16 +1,26 @@
 // Drop-in for the reference program cvc_exact2_xspace (cvc_exact2_xspace.c):
-// the conserved-vector-current Pi_mu_nu(x) of one point source, with the 120
-// propagators computed on the GPU instead of read from files.
+// the conserved-vector-current Pi_mu_nu(x) of one point source from 120
+// propagators, read from files or computed on the GPU.
 //
 //   cvc_exact2_xspace [-f input] [-w] [-h]
 //
 // -f  input file (default cvc.input), the reference's "key = value" lines
 //     (read_input_parser_cvc.l); used keys: T, L, LX, LY, LZ, kappa, mu,
 //     BCangleT/X/Y/Z, Nconf, gaugefilename_prefix, outfile_prefix,
-//     source_location, solver_precision, niter_max; and two keys of this
-//     program: gpu_precision (64, 32 or 16: the inner solver; default 64) and
-//     gpu_even_odd (0/1: even/odd preconditioned solves; default 0).  Other
-//     keys are accepted and ignored.
+//     source_location, solver_precision, niter_max, read_propagator,
+//     write_propagator, format, filename_prefix, filename_prefix2; and two keys
+//     of this program: gpu_precision (64, 32 or 16: the inner solver; default
+//     64) and gpu_even_odd (0/1: even/odd preconditioned solves; default 0).
+//     Other keys are accepted and ignored.
+//
+// read_propagator = yes (the reference's default) reads the up propagator of
+// shift s and spin-colour a from record 0 and the down propagator from record
+// 1 of the file get_filename(s, a, +1) names (cvc_exact2_xspace.c:395-431,
+// cvc_utils.c:2143-2197, formats 0, 2, 3 and 4).  read_propagator = no
+// computes them on the GPU (the reference's tmLQCD_invert branch,
+// cvc_exact2_xspace.c:437-541); with write_propagator = yes they are also
+// written to those files (up: write_propagator, down appended), so that a
+// later run with read_propagator = yes reads them back.
 // -w  write the position-space Ward identity to WI_X.<Nconf>.0000
 //     (cvc_exact2_xspace.c:928-1008).
 //
@@ -42,6 +52,8 @@ struct Input {
   int T = 0, LX = 0, LY = 0, LZ = 0, Nconf = 0, source_location = 0, niter_max = 1000;
   double kappa = 0., mu = 0., bc[4] = {1., 0., 0., 0.}, solver_precision = 1e-12;
   std::string gauge_prefix = "identity", out_prefix = "NA";
+  std::string prop_prefix = "prop.", prop_prefix2 = "prop.";   // default_input_values.h:28-29
+  int format = 0, read_propagator = 1, write_propagator = 0;   // default_input_values.h:23, 135-136
   int gpu_precision = 64, gpu_even_odd = 0;
 };
 
@@ -75,6 +87,11 @@ bool read_input(const char *fn, Input &in) {
     else if (k == "source_location") in.source_location = atoi(val);
     else if (k == "solver_precision") in.solver_precision = atof(val);
     else if (k == "niter_max") in.niter_max = atoi(val);
+    else if (k == "format") in.format = atoi(val);
+    else if (k == "filename_prefix") in.prop_prefix = val;
+    else if (k == "filename_prefix2") in.prop_prefix2 = val;
+    else if (k == "read_propagator" && (!strcmp(val, "yes") || !strcmp(val, "no"))) in.read_propagator = !strcmp(val, "yes");
+    else if (k == "write_propagator" && (!strcmp(val, "yes") || !strcmp(val, "no"))) in.write_propagator = !strcmp(val, "yes");
     else if (k == "gpu_precision") in.gpu_precision = atoi(val);
     else if (k == "gpu_even_odd") in.gpu_even_odd = atoi(val);
   }
@@ -117,6 +134,33 @@ double plaquette(const std::vector<double> &g, const int d[4]) {
           for (int k = 0; k < 3; k++) sum += (s[i][k] * std::conj(t[i][k])).real();
       }
   return sum / ((double)V * 18.);
+}
+
+// get_filename(filename, nu, sc, +1) (reference cvc_utils.c:2143-2197): the
+// propagator file of source shift nu (0-3: y + nu^, 4: y) and spin-colour sc;
+// empty for a format the reference does not name a file for
+std::string prop_filename(const Input &in, int nu, int sc) {
+  int isx[4] = {in.source_location / (in.LX * in.LY * in.LZ), (in.source_location % (in.LX * in.LY * in.LZ)) / (in.LY * in.LZ),
+                (in.source_location % (in.LY * in.LZ)) / in.LZ, in.source_location % in.LZ};
+  const int L[4] = {in.T, in.LX, in.LY, in.LZ};
+  char b[8400];
+  const char *p = in.prop_prefix.c_str(), *p2 = in.prop_prefix2.c_str();
+  switch (in.format) {
+    case 0: snprintf(b, sizeof(b), "%s.%.4d.%.2d.%.2d.inverted", p, in.Nconf, nu, sc); break;
+    case 2:
+    case 3:
+      if (nu != 4) isx[nu] = (isx[nu] + 1) % L[nu];
+      if (in.format == 2)
+        snprintf(b, sizeof(b), "%s.%.4d.t%.2dx%.2dy%.2dz%.2ds%.2dc%.2d.pmass.%s.inverted", p, in.Nconf, isx[0], isx[1],
+                 isx[2], isx[3], sc / 3, sc % 3, p2);
+      else
+        snprintf(b, sizeof(b), "%s.%.4d.t%.2dx%.2dy%.2dz%.2d.pmass.%s.%.2d.inverted", p, in.Nconf, isx[0], isx[1], isx[2],
+                 isx[3], p2, sc);
+      break;
+    case 4: snprintf(b, sizeof(b), "%s.%.4d.%.2d.inverted", p, in.Nconf, sc); break;
+    default: return std::string();
+  }
+  return b;
 }
 
 double now() {
@@ -189,8 +233,44 @@ int main(int argc, char **argv) {
   std::vector<double> conn((size_t)V * 32);
   double contact[8];
   cvchip_stats st;
+  memset(&st, 0, sizeof(st));
   const double t0 = now();
-  if (!rc) rc = cvchip_cvc_source(ctx, conn.data(), contact, src, in.solver_precision, in.niter_max, prec, delta, &st);
+  if (!rc && !in.read_propagator && !in.write_propagator) {
+    // propagators stay in HBM from the solves to the contraction
+    rc = cvchip_cvc_source(ctx, conn.data(), contact, src, in.solver_precision, in.niter_max, prec, delta, &st);
+  } else if (!rc) {
+    std::vector<std::vector<double>> store(120, std::vector<double>((size_t)V * 24));
+    std::vector<double *> props(120);
+    for (int k = 0; k < 120; k++) props[k] = store[k].data();
+    if (in.read_propagator) {
+      for (int k = 0; k < 120 && !rc; k++) {
+        const std::string name = prop_filename(in, (k % 60) / 12, k % 12);
+        if (name.empty()) {
+          fprintf(stderr, "[cvc_exact2_xspace] no propagator file name for format %d\n", in.format);
+          rc = CVCHIP_ERR_ARG;
+          break;
+        }
+        uint32_t ck[2];
+        fprintf(stdout, "# [read_lime_spinor] Reading Dirac-fermion field in LIME format from %s\n", name.c_str());
+        rc = cvchip_lime_read_spinor(name.c_str(), dims, k < 60 ? 0 : 1, in.kappa, props[k], ck);
+        if (!rc) fprintf(stdout, "[read_binary_spinor_data] The final checksum is %#x %#x\n", ck[0], ck[1]);
+      }
+    } else {
+      rc = cvchip_cvc_propagators(ctx, props.data(), src, in.solver_precision, in.niter_max, prec, delta, &st);
+      for (int k = 0; k < 60 && !rc; k++) {
+        const std::string name = prop_filename(in, k / 12, k % 12);
+        if (name.empty()) {
+          fprintf(stderr, "[cvc_exact2_xspace] no propagator file name for format %d\n", in.format);
+          rc = CVCHIP_ERR_ARG;
+          break;
+        }
+        uint32_t ck[2];
+        rc = cvchip_lime_write_propagator(name.c_str(), dims, props[k], 64, ck);
+        if (!rc) rc = cvchip_lime_write_spinor(name.c_str(), dims, props[60 + k], 1, 64, ck);
+      }
+    }
+    if (!rc) rc = cvchip_cvc_contract(ctx, conn.data(), contact, props.data(), src);
+  }
   if (rc) {
     fprintf(stderr, "[cvc_exact2_xspace] error %d: %s\n", rc, cvchip_last_error());
     if (ctx) cvchip_ctx_destroy(ctx);

@@ -82,10 +82,12 @@ int cvchip_ctx_set_solver_flags(cvchip_ctx *ctx, unsigned flags);
  * block-to-site traversal: 0 automatic, 1 time slices cut into 8 XCD chunks,
  * 2 one time slice per XCD walked in y blocks, 3 linear, 4 bricks of bt x bx x
  * yb x LZ sites marched along t per XCD.  Bits 8-15: the y-block height of
- * traversals 2 and 4 (0: the largest of 16, 8, 4, 2 dividing LY); bits 16-19
+ * traversals 2 and 4 (0: the largest of 16, 8, 4, 2 dividing LY; for the
+ * automatic bricks the largest of 8, 4, 2 with yb * N <= 96); bits 16-19
  * bt and 20-23 bx of traversal 4 (0: 4); bits 24-27 extra LDS per block in
  * 8 KiB units (caps the resident blocks).  Default 5: links in LDS,
- * streaming stores, automatic traversal (bricks 4x4x8 for N >= 8).  The operator output does not
+ * streaming stores, automatic traversal (bricks 4x4x8 for 8 <= N <= 12,
+ * 4x4x4 for N up to 24, 4x4x2 beyond).  The operator output does not
  * depend on it; the fused dot products of the solvers differ at rounding level. */
 int cvchip_ctx_set_dslash_policy(cvchip_ctx *ctx, int policy);
 const char *cvchip_last_error(void);
@@ -205,6 +207,13 @@ int cvchip_piq_orbits(const int dims[4], const double *piq, int max_orbits, doub
 int cvchip_cvc_source(cvchip_ctx *ctx, double *conn, double contact[8], const int src[4],
                       double sp, int niter_max, int prec, double delta,
                       cvchip_stats *stats);
+/* The 120 propagators of one source point into host fields props (the
+ * g_spinor_field order of cvchip_cvc_contract): the inversions of
+ * cvc_exact2_xspace.c:437-541 (tmLQCD_invert of a point source at y + s^ with
+ * the up operator, op_id 0, and the down operator, op_id 1), done on the GPU
+ * with the same solves as cvchip_cvc_source. */
+int cvchip_cvc_propagators(cvchip_ctx *ctx, double *const *props, const int src[4], double sp, int niter_max,
+                           int prec, double delta, cvchip_stats *stats);
 
 /* gamma_permutation / gamma_sign (global.h:143, filled by init_gamma,
  * cvc_utils.c:707-1520): the 16 gamma matrices as signed permutations of the
@@ -253,6 +262,25 @@ int cvchip_lime_read_gauge(const char *filename, const int dims[4], double *gaug
  * "scidac-checksum").  No reference counterpart (the path reads gauges only). */
 int cvchip_lime_write_gauge(const char *filename, const int dims[4], const double *gauge, int prec,
                             uint32_t checksum[2]);
+/* read_lime_spinor (propagator_io.c:771-840, with read_binary_spinor_data
+ * 344-405): the position-th scidac-binary-data record (0: up, 1: down in a
+ * two-flavour file; sites t z y x with x fastest, 24 big-endian reals, 64 or 32
+ * bit by the record length) into s[V][24] (the reference layout); an error if a
+ * following scidac-checksum record disagrees.  A file without such a record is
+ * read as CMI (read_cmi, propagator_io.c:842-892: host-order floats, sites
+ * x y z t with t fastest, times 2 kappa; kappa is used only there). */
+int cvchip_lime_read_spinor(const char *filename, const int dims[4], int position, double kappa, double *s,
+                            uint32_t checksum[2]);
+/* write_lime_spinor (propagator_io.c:582-647): "scidac-binary-data" (prec 64
+ * or 32) and "scidac-checksum" (write_checksum, 410-466), appended (append = 1)
+ * or in a new file. */
+int cvchip_lime_write_spinor(const char *filename, const int dims[4], const double *s, int append, int prec,
+                             uint32_t checksum[2]);
+/* write_propagator (propagator_io.c:125-133): a new file with the
+ * "etmc-propagator-format" record (write_propagator_format, 522-576, one
+ * flavour) followed by write_lime_spinor. */
+int cvchip_lime_write_propagator(const char *filename, const int dims[4], const double *s, int prec,
+                                 uint32_t checksum[2]);
 
 /* Gradient of L = sum_mu sum_{q in Q} Re Pi_{mu mu}(q) (Pi(q) as in cvchip_pi_q,
  * Q = {q : |q| <= qmax}, q_mu = 2 pi n_mu / L_mu with n_mu in (-L_mu/2, L_mu/2])

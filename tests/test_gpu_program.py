@@ -31,7 +31,8 @@ def test_program_reproduces_golden(chip, orc, tmp_path, variant):
     (tmp_path / "cvc.input").write_text(
         "T = 4\nL = 4\nkappa = 0.163\nmu = 0.01\nNconf = 1\n"
         f"gaugefilename_prefix = {tmp_path}/conf\nsource_location = {sl}\n"
-        "solver_precision = 1.e-26\nniter_max = 5000\nBCangleT = 1.\n# a comment\nformat = 0\n" + extra)
+        "solver_precision = 1.e-26\nniter_max = 5000\nBCangleT = 1.\n# a comment\nformat = 0\n"
+        "read_propagator = no\n" + extra)
     print(f"program {variant}", flush=True)
     p = subprocess.run([PROG, "-f", "cvc.input", "-w"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
@@ -56,3 +57,68 @@ def test_program_reproduces_golden(chip, orc, tmp_path, variant):
         c[nu] = (c[nu] + 1) % 4
         want[((c[0] * 4 + c[1]) * 4 + c[2]) * 4 + c[3], nu] -= gold["contact4"][nu]
     assert np.max(np.abs(w - want)) < 1e-8 * np.max(np.abs(want))
+
+
+def prop_name(fmt, s, a, src):
+    """get_filename(.., s, a, +1) of cvc_utils.c:2143-2197 at 4^4, prefix "prop." (the default)
+    and filename_prefix2 "lat4"."""
+    if fmt == 0:
+        return f"prop..0001.{s:02d}.{a:02d}.inverted"
+    c = list(src)
+    if s != 4:
+        c[s] = (c[s] + 1) % 4
+    return f"prop..0001.t{c[0]:02d}x{c[1]:02d}y{c[2]:02d}z{c[3]:02d}s{a // 3:02d}c{a % 3:02d}.pmass.lat4.inverted"
+
+
+@pytest.mark.parametrize("fmt", [0, 2])
+def test_program_propagator_files(chip, orc, tmp_path, fmt):
+    gold = np.load(os.path.join(HERE, "golden", "golden_v1.npz"), allow_pickle=False)
+    dims = (4, 4, 4, 4)
+    V = 256
+    g = chip.gauge_random(dims, 1, 0.3)
+    chip.lime_write_gauge(tmp_path / "conf.0001", dims, g)
+    src = tuple(int(v) for v in gold["conn4_src"])
+    sl = ((src[0] * 4 + src[1]) * 4 + src[2]) * 4 + src[3]
+    base = ("T = 4\nL = 4\nkappa = 0.163\nmu = 0.01\nNconf = 1\n"
+            f"gaugefilename_prefix = {tmp_path}/conf\nsource_location = {sl}\n"
+            f"solver_precision = 1.e-26\nniter_max = 5000\nformat = {fmt}\nfilename_prefix2 = lat4\n")
+    (tmp_path / "w.input").write_text(base + "read_propagator = no\nwrite_propagator = yes\n")
+    (tmp_path / "r.input").write_text(base)
+    print(f"program write_propagator format {fmt}", flush=True)
+    p = subprocess.run([PROG, "-f", "w.input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    conn_w, _ = chip.lime_read_contraction(tmp_path / "cvc2_v_x.0001", dims)
+    names = sorted(f.name for f in tmp_path.iterdir() if f.name.endswith(".inverted"))
+    assert names == sorted(prop_name(fmt, s, a, src) for s in range(5) for a in range(12))
+    (tmp_path / "cvc2_v_x.0001").unlink()
+    print(f"program read_propagator format {fmt}", flush=True)
+    p = subprocess.run([PROG, "-f", "r.input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.count("Reading Dirac-fermion field") == 120
+    conn_r, _ = chip.lime_read_contraction(tmp_path / "cvc2_v_x.0001", dims)
+    assert np.array_equal(conn_r, conn_w)
+    assert rel_err(conn_r, gold["conn4"]) < 1e-9
+    lat = orc.Lattice(dims, 0.163, 0.01)
+    for s, a in ((4, 0), (1, 7), (3, 11)):
+        c = list(src)
+        if s != 4:
+            c[s] = (c[s] + 1) % 4
+        b = np.zeros((V, 24))
+        b[((c[0] * 4 + c[1]) * 4 + c[2]) * 4 + c[3], 2 * a] = 1.0
+        up, _ = chip.lime_read_spinor(tmp_path / prop_name(fmt, s, a, src), dims, 0)
+        dn, _ = chip.lime_read_spinor(tmp_path / prop_name(fmt, s, a, src), dims, 1)
+        assert np.linalg.norm(lat.Q_phi(g, up, 0.01) - b) < 1e-10
+        assert np.linalg.norm(lat.Q_phi(g, dn, -0.01) - b) < 1e-10
+
+
+def test_cvc_propagators_match_cvc_source(chip):
+    """cvchip_cvc_propagators returns the propagators cvchip_cvc_source contracts: contracting
+    them with cvchip_cvc_contract gives the same Pi(x) and contact term bit for bit."""
+    dims = (4, 4, 4, 4)
+    ctx = chip.Context(dims, 0.163, 0.01, gauge=chip.gauge_random(dims, 2, 0.3))
+    src = (2, 1, 0, 3)
+    props, st = ctx.cvc_propagators(src, 1e-24, 5000)
+    assert st["converged"] == 120
+    conn_a, ct_a = ctx.cvc_contract(list(props), src)
+    conn_b, ct_b, _ = ctx.cvc_source(src, 1e-24, 5000)
+    assert np.array_equal(conn_a, conn_b) and np.array_equal(ct_a, ct_b)
