@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-source", action="store_true",
                     help="skip the whole-source run (120 propagators, FP16+norm inner solves, contraction)")
+    ap.add_argument("--no-48", action="store_true",
+                    help="skip the 48-rhs batched solve (12 spin-colour x 4 source positions)")
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
     ap.add_argument("--delta16", type=float, default=0.1, help="reliable-update factor of the FP16 mixed solver")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -160,6 +162,7 @@ def main():
                                (False, cvc_amd.PREC_F16, "mixed_f16"), (True, cvc_amd.PREC_F64, "eo_f64"),
                                (True, cvc_amd.PREC_F32, "eo_mixed_f32"), (True, cvc_amd.PREC_F16, "eo_mixed_f16")):
             x = ctx.field(N)
+            x.zero()                      # initial guess 0
             torch.cuda.synchronize()
             s0 = time.perf_counter()
             rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
@@ -181,6 +184,36 @@ def main():
         for k in ("eo_f64", "eo_mixed_f32", "eo_mixed_f16"):
             if extra.get(f"solve_{k}_s"):
                 extra[f"{k}_speedup"] = round(extra["solve_f64_s"] / extra[f"solve_{k}_s"], 3)
+
+    if not a.no_solve and not a.no_48:
+        # 48 rhs: 12 spin-colour x 4 source positions (seed 4) in one batched gamma5-hermitian
+        # solve (every operator launch applies D to all 48, links shared), FP32 inner iteration
+        # with FP64 reliable updates, then the four pion correlators from the batched propagators
+        rng = np.random.default_rng(4)
+        srcs = [tuple(int(rng.integers(0, L)) for L in dims) for _ in range(4)]
+        b48, x48 = ctx.field(48), ctx.field(48)
+        b48.point_sources(srcs)
+        for eo, name in ((False, "48rhs_mixed_f32"), (True, "48rhs_eo_mixed_f32")):
+            x48.zero()
+            torch.cuda.synchronize()
+            s0 = time.perf_counter()
+            rc, st = ctx.invert_her(x48, b48, 1e-20, 4000, prec=cvc_amd.PREC_F32, delta=a.delta, check=False, eo=eo)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - s0
+            extra[f"solve_{name}_s"] = round(el, 3)
+            extra[f"solve_{name}_iters"] = st["iterations"]
+            extra[f"solve_{name}_true_res2"] = st["true_res2_max"]
+            extra[f"solve_{name}_converged"] = st["converged"]
+            log(f"solve {name} (sources {srcs}): rc={rc} {el:.2f}s {st}")
+        one = ctx.field(12)
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        for k in range(4):
+            one.copy_columns(0, x48, 12 * k, 12)
+            ctx.field_contract_twopoint(5, 5, one, one, src=srcs[k])
+        extra["twopoint_48rhs_s"] = round(time.perf_counter() - s0, 4)
+        log(f"four pion correlators from the 48 propagators: {extra['twopoint_48rhs_s']}s")
+        del b48, x48, one
 
     if not a.no_solve and not a.no_source:
         # one source point: 60 up + 60 down propagators (10 batches of 12, even/odd

@@ -25,7 +25,8 @@ EXPORTS = [
     "cvchip_ctx_set_solver_flags", "cvchip_piq_orbits",
     "cvchip_lime_write_contraction", "cvchip_lime_read_contraction", "cvchip_lime_read_gauge",
     "cvchip_lime_write_gauge", "cvchip_lime_read_spinor", "cvchip_lime_write_spinor",
-    "cvchip_lime_write_propagator", "cvchip_cvc_propagators", "cvchip_gamma_tables", "cvchip_contract_twopoint",
+    "cvchip_lime_write_propagator", "cvchip_cvc_propagators", "cvchip_field_point_sources", "cvchip_field_zero",
+    "cvchip_field_copy_columns", "cvchip_gamma_tables", "cvchip_contract_twopoint",
     "cvchip_field_contract_twopoint",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
@@ -112,6 +113,9 @@ def lib():
         L.cvchip_lime_read_spinor.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_int, C.c_double, _dp, _u32p]
         L.cvchip_lime_write_spinor.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, C.c_int, _u32p]
         L.cvchip_lime_write_propagator.argtypes = [C.c_char_p, C.POINTER(C.c_int), _dp, C.c_int, _u32p]
+        L.cvchip_field_zero.argtypes = [_vp, _vp]
+        L.cvchip_field_point_sources.argtypes = [_vp, _vp, C.c_int, C.POINTER(C.c_int)]
+        L.cvchip_field_copy_columns.argtypes = [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int]
         L.cvchip_cvc_propagators.argtypes = [_vp, C.POINTER(_dp), C.POINTER(C.c_int), C.c_double, C.c_int, C.c_int,
                                              C.c_double, C.POINTER(Stats)]
         L.cvchip_gamma_tables.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -264,6 +268,18 @@ class Field:
 
     def copy_from(self, src: "Field"):
         _check(lib().cvchip_field_copy(self.ctx._h, self._h, src._h))
+
+    def zero(self):
+        _check(lib().cvchip_field_zero(self.ctx._h, self._h))
+
+    def point_sources(self, coords):
+        """Point sources of len(coords) source points (t, x, y, z): column 12 k + 3 spin + colour."""
+        flat = [int(v) for c in coords for v in c]
+        _check(lib().cvchip_field_point_sources(self.ctx._h, self._h, len(coords), (C.c_int * len(flat))(*flat)))
+
+    def copy_columns(self, dst_col0, src: "Field", src_col0, n):
+        """self[:, dst_col0:dst_col0+n] = src[:, src_col0:src_col0+n] (FP64 sets)."""
+        _check(lib().cvchip_field_copy_columns(self.ctx._h, self._h, int(dst_col0), src._h, int(src_col0), int(n)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -152,11 +152,118 @@ __global__ void k_fill_random(double2 *__restrict__ f, uint64_t n, uint64_t seed
   f[i] = make_double2(mix_uniform(seed * 0x100000001B3ull + 2 * i), mix_uniform(seed * 0x100000001B3ull + 2 * i + 1));
 }
 
+// Point sources of several source points (the reference's
+// source[_GSI(ix) + 2 ia] = 1, cvc_exact2_xspace.c:521-524): column 12 k + a of
+// the N = 12 nsrc columns holds the unit vector of spin-colour a at site[k].
+constexpr int kMaxPointSources = 21;   // 12 * 21 = 252 columns (at most 256)
+struct PointSites {
+  uint32_t site[kMaxPointSources];
+};
+
+__global__ void k_point_sources_n(double2 *__restrict__ b, uint32_t V, uint32_t N, PointSites ps) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;   // (c V + site) N + r
+  if (i >= (uint64_t)12 * V * N) return;
+  const uint32_t r = (uint32_t)(i % N);
+  const uint64_t cs = i / N;
+  const uint32_t site = (uint32_t)(cs % V), c = (uint32_t)(cs / V);
+  b[i] = make_double2((site == ps.site[r / 12] && c == r % 12) ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void k_zero16(uint4 *__restrict__ p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// dst[.., d0 + r] = src[.., s0 + r] for r < n (FP64 sets of the same volume)
+__global__ void k_copy_columns(double2 *__restrict__ dst, const double2 *__restrict__ src, uint32_t V, uint32_t ND,
+                               uint32_t NS, uint32_t d0, uint32_t s0, uint32_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)12 * V * n) return;
+  const uint32_t r = (uint32_t)(i % n);
+  const uint64_t cs = i / n;   // c V + site
+  dst[cs * ND + d0 + r] = src[cs * NS + s0 + r];
+}
+
 }  // namespace cvchip
 
 using namespace cvchip;
 
 // =================================================================== C ABI
+
+extern "C" int cvchip_field_zero(cvchip_ctx *c, cvchip_field *fh) {
+  if (!c || !fh) {
+    set_error("NULL argument to cvchip_field_zero");
+    return CVCHIP_ERR_NULL;
+  }
+  Field &f = fh->f;
+  const uint64_t n = f.bytes / 16;   // every set is a whole number of 16-byte planes
+  if (n) hipLaunchKernelGGL(k_zero16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (uint4 *)f.data, n);
+  if (f.norm) {
+    const uint64_t m = (uint64_t)f.V * f.nrhs;   // one float per (site, rhs): zero scale
+    hipLaunchKernelGGL(k_zero16, dim3((unsigned)((m / 4 + 255) / 256)), dim3(256), 0, c->stream, (uint4 *)f.norm, m / 4);
+    if (m % 4) CVC_HIP(hipMemsetAsync(f.norm + (m / 4) * 4, 0, sizeof(float) * (m % 4), c->stream));
+  }
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_field_point_sources(cvchip_ctx *c, cvchip_field *fh, int nsrc, const int *coords) {
+  if (!c || !fh || !coords) {
+    set_error("NULL argument to cvchip_field_point_sources");
+    return CVCHIP_ERR_NULL;
+  }
+  Field &f = fh->f;
+  if (f.prec != CVCHIP_PREC_F64 || f.V != c->lat.V || nsrc < 1 || nsrc > kMaxPointSources || f.nrhs != 12 * nsrc) {
+    set_error("point sources need a full-lattice FP64 set of 12 x nsrc columns, 1 <= nsrc <= %d (nsrc %d, %d columns)",
+              kMaxPointSources, nsrc, f.nrhs);
+    return CVCHIP_ERR_ARG;
+  }
+  const int L[4] = {c->lat.T, c->lat.LX, c->lat.LY, c->lat.LZ};
+  PointSites ps{};
+  for (int k = 0; k < nsrc; k++) {
+    const int *x = coords + 4 * k;
+    for (int m = 0; m < 4; m++)
+      if (x[m] < 0 || x[m] >= L[m]) {
+        set_error("source %d coordinate %d = %d outside 0..%d", k, m, x[m], L[m] - 1);
+        return CVCHIP_ERR_ARG;
+      }
+    ps.site[k] = (uint32_t)((((int64_t)x[0] * L[1] + x[1]) * L[2] + x[2]) * L[3] + x[3]);
+  }
+  const uint64_t n = (uint64_t)12 * f.V * f.nrhs;
+  hipLaunchKernelGGL(k_point_sources_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     (double2 *)f.data, (uint32_t)f.V, (uint32_t)f.nrhs, ps);
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
+}
+
+extern "C" int cvchip_field_copy_columns(cvchip_ctx *c, cvchip_field *dst, int dst_col0, const cvchip_field *src,
+                                         int src_col0, int n) {
+  if (!c || !dst || !src) {
+    set_error("NULL argument to cvchip_field_copy_columns");
+    return CVCHIP_ERR_NULL;
+  }
+  const Field &d = dst->f, &s = src->f;
+  if (d.prec != CVCHIP_PREC_F64 || s.prec != CVCHIP_PREC_F64 || d.V != s.V) {
+    set_error("column copies need FP64 sets of the same volume");
+    return CVCHIP_ERR_ARG;
+  }
+  if (n < 0 || dst_col0 < 0 || src_col0 < 0 || dst_col0 + n > d.nrhs || src_col0 + n > s.nrhs ||
+      (d.data == s.data && dst_col0 < src_col0 + n && src_col0 < dst_col0 + n)) {
+    set_error("column ranges %d+%d of %d and %d+%d of %d are invalid or overlap", dst_col0, n, d.nrhs, src_col0, n,
+              s.nrhs);
+    return CVCHIP_ERR_ARG;
+  }
+  if (n == 0) return CVCHIP_OK;
+  const uint64_t tot = (uint64_t)12 * d.V * n;
+  hipLaunchKernelGGL(k_copy_columns, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, (double2 *)d.data,
+                     (const double2 *)s.data, (uint32_t)d.V, (uint32_t)d.nrhs, (uint32_t)s.nrhs, (uint32_t)dst_col0,
+                     (uint32_t)src_col0, (uint32_t)n);
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
+}
 
 extern "C" int cvchip_field_fill_random(cvchip_ctx *c, cvchip_field *fh, uint64_t seed) {
   if (!c || !fh) return CVCHIP_ERR_NULL;

@@ -139,6 +139,18 @@ int cvchip_field_download(cvchip_ctx *ctx, const cvchip_field *f, double *host, 
 int cvchip_field_fill_random(cvchip_ctx *ctx, cvchip_field *f, uint64_t seed);
 /* dst = src, converting precision (FP64 / FP32 / FP16+norm); equal nrhs. */
 int cvchip_field_copy(cvchip_ctx *ctx, cvchip_field *dst, const cvchip_field *src);
+/* f = 0 (every component, and the FP16 scales). */
+int cvchip_field_zero(cvchip_ctx *ctx, cvchip_field *f);
+/* Point sources of nsrc source points coords[4 k .. 4 k + 3] = (t, x, y, z) into an
+ * FP64 set of 12 nsrc columns (nsrc <= 21): column 12 k + 3 spin + colour is the
+ * unit vector source[_GSI(site_k) + 2 (3 spin + colour)] = 1 of
+ * cvc_exact2_xspace.c:521-524. */
+int cvchip_field_point_sources(cvchip_ctx *ctx, cvchip_field *f, int nsrc, const int *coords);
+/* dst columns dst_col0 .. dst_col0 + n - 1 = src columns src_col0 .. (FP64 sets of
+ * one volume; ranges inside the sets and not overlapping).  No reference
+ * counterpart: it selects one source's 12 columns of a batched set. */
+int cvchip_field_copy_columns(cvchip_ctx *ctx, cvchip_field *dst, int dst_col0, const cvchip_field *src,
+                              int src_col0, int n);
 /* out = D(mutm) in (gamma5_out: gamma5 D in) on all rhs of the set; precision
  * of the sets selects the kernel. */
 int cvchip_field_dslash(cvchip_ctx *ctx, cvchip_field *out, const cvchip_field *in,
