@@ -214,12 +214,13 @@ int contract_fields(cvchip_ctx *c, double *conn, double contact[8], const Field 
   uint32_t *dysp = nullptr;
   int rc = 0;
   auto fail = [&](hipError_t e, const char *w) { return hip_fail(e, w, __FILE__, __LINE__); };
+  const size_t conn_bytes = sizeof(double2) * 16 * (size_t)V, psi_bytes = sizeof(double2) * 24 * 12 * (size_t)V;
   hipError_t e = hipMalloc(&dL, sizeof(SrcLinks));
-  if (e == hipSuccess) e = hipMalloc(&dconn, sizeof(double2) * 16 * (size_t)V);
-  if (e == hipSuccess) e = hipMalloc(&psi, sizeof(double2) * 24 * 12 * (size_t)V);
   if (e == hipSuccess) e = hipMalloc(&dct, sizeof(double2) * 96);
   if (e == hipSuccess) e = hipMalloc(&dysp, sizeof(uint32_t) * 4);
   if (e != hipSuccess) rc = fail(e, "contraction buffers");
+  if (!rc) rc = pool_get(c, (void **)&dconn, conn_bytes);   // the large buffers come from the context's pool
+  if (!rc) rc = pool_get(c, (void **)&psi, psi_bytes);
   if (!rc) {
     hipLaunchKernelGGL(k_src_links, dim3(1), dim3(64), 0, c->stream, dL, (const double2 *)c->gauge64, V, y, P);
     e = hipMemcpyAsync(&hL, dL, sizeof(SrcLinks), hipMemcpyDeviceToHost, c->stream);
@@ -255,8 +256,8 @@ int contract_fields(cvchip_ctx *c, double *conn, double contact[8], const Field 
     }
   }
   if (dL) (void)hipFree(dL);
-  if (dconn) (void)hipFree(dconn);
-  if (psi) (void)hipFree(psi);
+  pool_put(c, dconn, conn_bytes);
+  pool_put(c, psi, psi_bytes);
   if (dct) (void)hipFree(dct);
   if (dysp) (void)hipFree(dysp);
   return rc;

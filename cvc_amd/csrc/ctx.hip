@@ -58,7 +58,50 @@ static size_t elem_bytes(int prec) {
   return prec == CVCHIP_PREC_F64 ? sizeof(double2) : prec == CVCHIP_PREC_F32 ? sizeof(float2) : sizeof(__half2);
 }
 
-int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites) {
+int pool_get(cvchip_ctx *c, void **p, size_t bytes) {
+  for (size_t k = c->idle.size(); k-- > 0;)
+    if (c->idle[k].second == bytes) {   // the most recently freed buffer of this size
+      *p = c->idle[k].first;
+      c->idle_bytes -= bytes;
+      c->idle.erase(c->idle.begin() + (std::ptrdiff_t)k);
+      return 0;
+    }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && !c->idle.empty()) {
+    (void)hipGetLastError();   // clear the failed allocation's error, then retry without idle buffers
+    pool_release(c);
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return CVCHIP_ERR_OOM;
+  }
+  return 0;
+}
+
+// Buffers return to the pool in stream order: every use of a context's
+// temporaries is enqueued on the context's one stream, so a later reuse
+// cannot overtake an earlier kernel.
+void pool_put(cvchip_ctx *c, void *p, size_t bytes) {
+  if (!p) return;
+  c->idle.emplace_back(p, bytes);
+  c->idle_bytes += bytes;
+  while (c->idle_bytes > kPoolIdleMax && !c->idle.empty()) {   // oldest first
+    (void)hipFree(c->idle.front().first);
+    c->idle_bytes -= c->idle.front().second;
+    c->idle.erase(c->idle.begin());
+  }
+}
+
+void pool_release(cvchip_ctx *c) {
+  for (auto &b : c->idle) (void)hipFree(b.first);
+  c->idle.clear();
+  c->idle_bytes = 0;
+}
+
+int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites, bool pooled) {
   if (nrhs <= 0 || nrhs > 256) {
     set_error("nrhs %d outside 1..256", nrhs);
     return CVCHIP_ERR_ARG;
@@ -75,16 +118,29 @@ int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites) {
   f.prec = prec;
   f.V = sites > 0 ? sites : c->lat.V;
   f.bytes = (size_t)12 * f.V * nrhs * elem_bytes(prec);
-  CVC_HIP(hipMalloc(&f.data, f.bytes));
-  if (prec == CVCHIP_PREC_F16) CVC_HIP(hipMalloc(&f.norm, sizeof(float) * f.V * nrhs));
+  if (!pooled) {
+    f.pool = nullptr;
+    CVC_HIP(hipMalloc(&f.data, f.bytes));
+    if (prec == CVCHIP_PREC_F16) CVC_HIP(hipMalloc(&f.norm, sizeof(float) * f.V * nrhs));
+    return 0;
+  }
+  f.pool = c;
+  CVC_TRY(pool_get(c, &f.data, f.bytes));
+  if (prec == CVCHIP_PREC_F16) CVC_TRY(pool_get(c, (void **)&f.norm, sizeof(float) * f.V * nrhs));
   return 0;
 }
 
 void field_free(Field &f) {
-  if (f.data) (void)hipFree(f.data);
-  if (f.norm) (void)hipFree(f.norm);
+  if (f.pool) {
+    pool_put(f.pool, f.data, f.bytes);
+    if (f.norm) pool_put(f.pool, f.norm, sizeof(float) * f.V * f.nrhs);
+  } else {
+    if (f.data) (void)hipFree(f.data);
+    if (f.norm) (void)hipFree(f.norm);
+  }
   f.data = nullptr;
   f.norm = nullptr;
+  f.pool = nullptr;
 }
 
 // ------------------------------------------------------------- kernels
@@ -332,6 +388,7 @@ extern "C" int cvchip_ctx_create(cvchip_ctx **ctx, const int dims[4], const doub
 extern "C" int cvchip_ctx_destroy(cvchip_ctx *c) {
   if (!c) return CVCHIP_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  pool_release(c);
   if (c->gauge64) (void)hipFree(c->gauge64);
   if (c->gauge64r12) (void)hipFree(c->gauge64r12);
   if (c->gauge32r12) (void)hipFree(c->gauge32r12);
@@ -384,12 +441,23 @@ extern "C" int cvchip_ctx_set_dslash_policy(cvchip_ctx *c, int policy) {
   if (!c) return CVCHIP_ERR_NULL;
   const int mode = (policy >> 4) & 15, yb = (policy >> 8) & 255;
   const int bt = (policy >> 16) & 15, bx = (policy >> 20) & 15;
-  if (policy < 0 || policy >= (1 << 28) || mode > 4) {
-    set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..4, 8-15 y-block, "
-              "16-19 brick t, 20-23 brick x, 24-27 LDS padding", policy);
+  if (policy < 0 || policy >= (1 << 28) || mode > 5) {
+    set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..5, 8-15 y-block, "
+              "16-19 brick t (traversal 5: log2 of the column's z extent), 20-23 brick x, 24-27 LDS padding",
+              policy);
     return CVCHIP_ERR_ARG;
   }
-  if ((yb && c->lat.LY % yb) || (bt && c->lat.T % bt) || (bx && c->lat.LX % bx)) {
+  if (mode == 5) {
+    // column extents: powers of two dividing the lattice
+    const int zb = bt ? 1 << bt : 8;
+    const int cy = yb ? yb : 4, cx = bx ? bx : 4;
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (bt > 5 || !pow2(cy) || !pow2(cx) || c->lat.LY % cy || c->lat.LX % cx || c->lat.LZ % zb) {
+      set_error("operator policy 0x%x: column extents x %d, y %d, z %d must be powers of two dividing LX=%d, "
+                "LY=%d, LZ=%d", policy, cx, cy, zb, c->lat.LX, c->lat.LY, c->lat.LZ);
+      return CVCHIP_ERR_ARG;
+    }
+  } else if ((yb && c->lat.LY % yb) || (bt && c->lat.T % bt) || (bx && c->lat.LX % bx)) {
     set_error("operator policy 0x%x: block extents must divide the lattice (T=%d LX=%d LY=%d)", policy,
               c->lat.T, c->lat.LX, c->lat.LY);
     return CVCHIP_ERR_ARG;
@@ -479,7 +547,7 @@ int download_aos(cvchip_ctx *c, const Field &f, double *host, int rhs0, int n) {
 extern "C" int cvchip_field_create(cvchip_ctx *c, int nrhs, int prec, cvchip_field **out) {
   if (!c || !out) return CVCHIP_ERR_NULL;
   cvchip_field *f = new cvchip_field();
-  int rc = field_alloc(c, f->f, nrhs, prec);
+  int rc = field_alloc(c, f->f, nrhs, prec, 0, false);
   if (rc) {
     field_free(f->f);
     delete f;

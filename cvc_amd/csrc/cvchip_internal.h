@@ -97,6 +97,7 @@ struct Field {
   void *data = nullptr;      // cplx<R>[12][V][N]
   float *norm = nullptr;     // FP16 storage: per (site, rhs) scale
   size_t bytes = 0;
+  cvchip_ctx *pool = nullptr;  // context whose idle-buffer pool takes the memory back (temporaries)
 };
 
 struct Scratch {
@@ -133,7 +134,11 @@ struct cvchip_ctx {
   // reduction scratch
   cvchip::Scratch red;
   cvchip::Scratch red2;   // first-stage sums of the two-stage partial reduction
-  // per-context solver workspaces are allocated per call
+  // idle device buffers of freed temporaries (solver, propagator and
+  // contraction workspaces), reused by exact size: repeated calls allocate
+  // nothing; at most kPoolIdleMax bytes stay idle
+  std::vector<std::pair<void *, size_t>> idle;
+  size_t idle_bytes = 0;
 };
 
 namespace cvchip {
@@ -142,8 +147,16 @@ int ensure_stage(cvchip_ctx *c, size_t bytes);
 int ensure_red(cvchip_ctx *c, size_t bytes);
 int ensure_red2(cvchip_ctx *c, size_t bytes);
 // sites: 0 for the whole lattice, or the half lattice V/2 of the even/odd solver
-int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites = 0);
+// Temporaries of the library (pooled); user sets of cvchip_field_create pass
+// pooled = false, so they own their memory outright.
+int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites = 0, bool pooled = true);
 void field_free(Field &f);
+// device memory from the context's idle pool (or hipMalloc; on failure the
+// pool is released and the allocation retried); back into the pool
+constexpr size_t kPoolIdleMax = (size_t)96 << 30;
+int pool_get(cvchip_ctx *c, void **p, size_t bytes);
+void pool_put(cvchip_ctx *c, void *p, size_t bytes);
+void pool_release(cvchip_ctx *c);
 
 // host AoS [V][24] (nrhs contiguous fields) <-> device field rhs range
 int upload_aos(cvchip_ctx *c, Field &f, const double *host, int rhs0, int n);

@@ -33,10 +33,16 @@ struct DParams {
   // Mode 4: bricks of bt time slices x bx x-planes x yb y-rows (all z); the
   // bricks of one (x, y) column follow each other in t on one XCD, so the
   // sites in flight on an XCD form a compact 4D region.
+  // Mode 5: columns of bx x-planes x yb y-rows x zb z-sites (powers of two)
+  // swept along the whole t extent, one column at a time per XCD, with the
+  // column's sites ordered z, y, x within a time slice: the t neighbour of a
+  // site lies one cross-section (bx yb zb sites, a few blocks) ahead in the
+  // sweep, inside the XCD's L2 window even for 12 rhs in FP64.
   uint32_t csz, bpc;
   int mode;
   uint32_t yb, runlen, bpr, nrun;
   uint32_t bt, bx, ncx, bpb, bpcol;
+  uint32_t zb, ncy, ncz, lzb, lyz, lA, colsz, sTt, sXt;
   int nt;   // streaming output stores
   // Even/odd operator (EO kernels): the traversal runs over the half lattice
   // (T, LX, LY, LZ/2); half-site h of parity par is the lexicographic site
@@ -253,8 +259,16 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     }
   };
   const uint32_t S = blockDim.x / (uint32_t)P.N;
-  uint32_t s0, s_end;
-  if (P.mode == 4) {
+  uint32_t s0 = 0, s_end = 0;   // modes 0-4: the block's contiguous traversal range
+  uint32_t jb = 0, cbase = 0;   // mode 5: first column-local index, column origin
+  if (P.mode == 5) {
+    const uint32_t k = blockIdx.x / kXcds;
+    const uint32_t col = (k / P.bpcol) * kXcds + blockIdx.x % kXcds;
+    jb = (k - (k / P.bpcol) * P.bpcol) * S;
+    const uint32_t cz = col % P.ncz, q = col / P.ncz;
+    const uint32_t cy = q % P.ncy, cx = q / P.ncy;
+    cbase = ((cx * P.bx) * (uint32_t)P.LY + cy * P.yb) * P.LZt + cz * P.zb;
+  } else if (P.mode == 4) {
     const uint32_t k = blockIdx.x / kXcds;
     const uint32_t col = k / P.bpcol, rem = k - col * P.bpcol;
     const uint32_t brick = rem / P.bpb, rem2 = rem - brick * P.bpb;
@@ -289,8 +303,25 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     s0 = chunk * P.csz + blk * S;
     s_end = min(s0 + S, (chunk + 1) * P.csz);
   }
+  // traversal index (site, or half-site for EO) of block slot sl; false past
+  // the end of the block's range
+  auto slot_v = [&](uint32_t sl, uint32_t &v) -> bool {
+    if (P.mode == 5) {
+      const uint32_t j = jb + sl;
+      v = 0;
+      if (j >= P.colsz) return false;
+      const uint32_t rem = j & ((1u << P.lA) - 1u);
+      v = cbase + (j >> P.lA) * P.sTt + (rem >> P.lyz) * P.sXt + ((rem >> P.lzb) & (P.yb - 1u)) * P.LZt +
+          (rem & (P.zb - 1u));
+      return true;
+    }
+    v = s0 + sl;
+    return v < s_end;
+  };
   const uint32_t r = threadIdx.x % (uint32_t)P.N;
-  const uint32_t vs = s0 + threadIdx.x / (uint32_t)P.N;   // traversal index (site, or half-site for EO)
+  const uint32_t sl = threadIdx.x / (uint32_t)P.N;
+  uint32_t vs;
+  const bool active = slot_v(sl, vs);
   const uint32_t i = vs * (uint32_t)P.N + r;
   double dv[3] = {0., 0., 0.};
   __shared__ C Ul[LDSL ? 8 * KL * LDS_SITES : 1];
@@ -298,9 +329,10 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     const uint32_t tot = 8u * KL * S;
     for (uint32_t e = threadIdx.x; e < tot; e += blockDim.x) {
       const uint32_t hopi = e / (KL * S), rem = e - hopi * (KL * S);
-      const uint32_t k = rem / S, sl = rem - k * S;
-      if (s0 + sl >= s_end) continue;
-      const uint32_t site = lex_of(s0 + sl);
+      const uint32_t k = rem / S, sle = rem - k * S;
+      uint32_t ve;
+      if (!slot_v(sle, ve)) continue;
+      const uint32_t site = lex_of(ve);
       const int mu = (int)(hopi >> 1);
       uint32_t ls = site;
       if ((hopi & 1u) == 0) {          // backward hop: link at site - mu
@@ -309,7 +341,7 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
         const uint32_t crd = (site / str) % L;
         ls = crd == 0 ? site + (L - 1) * str : site - str;
       }
-      Ul[(hopi * KL + k) * LDS_SITES + sl] = U[((size_t)mu * KL + k) * P.V + ls];
+      Ul[(hopi * KL + k) * LDS_SITES + sle] = U[((size_t)mu * KL + k) * P.V + ls];
     }
     __syncthreads();
   }
@@ -329,7 +361,7 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     __syncthreads();
   }
   const NbLoader<IOI, LDSS> ld{in, Sp, (uint32_t)P.N, r, s0, S};
-  if (vs < s_end) {
+  if (active) {
     const uint32_t s = lex_of(vs);
     const uint32_t t = s / P.sT;
     uint32_t rem = s - t * P.sT;
@@ -341,7 +373,6 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     C acc[12];
 #pragma unroll
     for (int c = 0; c < 12; c++) acc[c] = mk<R>(0, 0);
-    const uint32_t sl = vs - s0;
 #ifndef CVC_HOP_ORDER
 #define CVC_HOP_ORDER 0
 #endif
@@ -460,11 +491,21 @@ struct Traversal {
   int mode;
   uint32_t yb, runlen, bpr, nrun;
   uint32_t bt, bx, ncx, bpb, bpcol;
+  uint32_t zb, ncy, ncz, lzb, lyz, lA, colsz;
 };
 
+static int log2_exact(uint32_t v) {   // -1 unless v is a power of two
+  if (v == 0 || (v & (v - 1))) return -1;
+  int l = 0;
+  while ((1u << l) < v) l++;
+  return l;
+}
+
 // Block-to-site mapping; the policy's bits 4-7 pick the mode (0: automatic),
-// bits 8-15 the y-block height of modes 2 and 4 (0: automatic), bits 16-19
-// and 20-23 the brick's t and x extents of mode 4 (0: 4).
+// bits 8-15 the y-block height of modes 2, 4 and 5 (0: automatic), bits 16-19
+// and 20-23 the brick's t and x extents of mode 4 (0: 4); for mode 5, bits
+// 16-19 are log2 of the column's z extent (0: 8) and bits 20-23 its x extent
+// (0: 4).
 static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
   Lattice L = c->lat;
   if (eo) {   // the half lattice of one parity: LZ/2 half-sites per z line
@@ -494,6 +535,32 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
     }
   }
   Traversal t{};
+  if (want == 5) {
+    const uint32_t cyb = (c->dslash_policy >> 8) & 255 ? yb : 4u;
+    const uint32_t lz = (c->dslash_policy >> 16) & 15;
+    const uint32_t czb = lz ? (1u << lz) : 8u;
+    const uint32_t cbx = (c->dslash_policy >> 20) & 15 ? (c->dslash_policy >> 20) & 15 : 4u;
+    const int l1 = log2_exact(cbx), l2 = log2_exact(cyb), l3 = log2_exact(czb);
+    if (l1 >= 0 && l2 >= 0 && l3 >= 0 && L.LX % (int)cbx == 0 && L.LY % (int)cyb == 0 && L.LZ % (int)czb == 0) {
+      const uint32_t ncol = (uint32_t)(L.LX / (int)cbx) * (uint32_t)(L.LY / (int)cyb) * (uint32_t)(L.LZ / (int)czb);
+      if (ncol % kXcds == 0) {
+        t.mode = 5;
+        t.bx = cbx;
+        t.yb = cyb;
+        t.zb = czb;
+        t.ncy = (uint32_t)L.LY / cyb;
+        t.ncz = (uint32_t)L.LZ / czb;
+        t.lzb = (uint32_t)l3;
+        t.lyz = (uint32_t)(l2 + l3);
+        t.lA = (uint32_t)(l1 + l2 + l3);
+        t.colsz = (uint32_t)L.T << t.lA;
+        t.bpcol = (t.colsz + S - 1) / S;
+        t.nblk = ncol * t.bpcol;
+        return t;
+      }
+    }
+    want = 4;   // a column request that does not tile the lattice falls back to bricks
+  }
   uint32_t bt = (uint32_t)((c->dslash_policy >> 16) & 15), bx = (uint32_t)((c->dslash_policy >> 20) & 15);
   if (bt == 0) bt = 4;
   if (bx == 0) bx = 4;
@@ -569,6 +636,15 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   P.ncx = tr.ncx;
   P.bpb = tr.bpb;
   P.bpcol = tr.bpcol;
+  P.zb = tr.zb;
+  P.ncy = tr.ncy;
+  P.ncz = tr.ncz;
+  P.lzb = tr.lzb;
+  P.lyz = tr.lyz;
+  P.lA = tr.lA;
+  P.colsz = tr.colsz;
+  P.sXt = (uint32_t)c->lat.LY * P.LZt;
+  P.sTt = (uint32_t)c->lat.LX * P.sXt;
   // streaming stores only for outputs that cannot stay in the 256 MB MALL
   // (bit 3: for every output)
   P.nt = (((c->dslash_policy >> 2) & 1) && out.bytes >= ((size_t)256 << 20)) || ((c->dslash_policy >> 3) & 1);
@@ -588,7 +664,7 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   dim3 g(nblk), b(B);
   const bool multi = B / N <= LDS_SITES;
   const bool ldsl = multi && (c->dslash_policy & 1);
-  const bool ldss = !EO && multi && (c->dslash_policy & 2);
+  const bool ldss = !EO && multi && (c->dslash_policy & 2) && tr.mode != 5;
   // bits 24-27: extra LDS per block in 8 KiB units (caps the blocks per CU)
   const size_t pad = (size_t)((c->dslash_policy >> 24) & 15) * 8192;
   const size_t dyn = (ldss ? (size_t)(B / N + 2) * N * 12 * sizeof(cplx<R>) : 0) + pad;

@@ -81,10 +81,12 @@ int cvchip_ctx_set_solver_flags(cvchip_ctx *ctx, unsigned flags);
  * every output.  Bits 4-7,
  * block-to-site traversal: 0 automatic, 1 time slices cut into 8 XCD chunks,
  * 2 one time slice per XCD walked in y blocks, 3 linear, 4 bricks of bt x bx x
- * yb x LZ sites marched along t per XCD.  Bits 8-15: the y-block height of
- * traversals 2 and 4 (0: the largest of 16, 8, 4, 2 dividing LY; for the
- * automatic bricks the largest of 8, 4, 2 with yb * N <= 96); bits 16-19
- * bt and 20-23 bx of traversal 4 (0: 4); bits 24-27 extra LDS per block in
+ * yb x LZ sites marched along t per XCD, 5 columns of bx x yb x zb sites
+ * (powers of two) swept along t per XCD.  Bits 8-15: the y-block height of
+ * traversals 2, 4 and 5 (0: the largest of 16, 8, 4, 2 dividing LY; for the
+ * automatic bricks the largest of 8, 4, 2 with yb * N <= 96; 4 for traversal
+ * 5); bits 16-19 bt of traversal 4 (0: 4) or log2 zb of traversal 5 (0: zb 8)
+ * and 20-23 bx of traversals 4 and 5 (0: 4); bits 24-27 extra LDS per block in
  * 8 KiB units (caps the resident blocks).  Default 5: links in LDS,
  * streaming stores, automatic traversal (bricks 4x4x8 for 8 <= N <= 12,
  * 4x4x4 for N up to 24, 4x4x2 beyond).  The operator output does not

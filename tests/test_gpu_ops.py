@@ -67,7 +67,9 @@ def test_dslash_multi_rhs(chip, orc, nrhs):
 # staging bits 0-1, streaming stores bits 2-3, traversal mode bits 4-7, y-block
 # bits 8-15, brick t / x extents bits 16-19 / 20-23, LDS padding bits 24-27
 POLICIES = [0x00, 0x01, 0x03, 0x10, 0x11, 0x13, 0x21, 0x23, 0x221, 0x421, 0x30, 0x31,
-            0x05, 0x08, 0x0829, 0x440141, 0x42014d, 0x2000001, 0x3440149, 0x45]
+            0x05, 0x08, 0x0829, 0x440141, 0x42014d, 0x2000001, 0x3440149, 0x45,
+            # t-swept columns (x, y, z extents 1x1x2, 2x2x2; 2x4x2 does not tile into 8 XCDs: bricks)
+            0x110155, 0x210255, 0x210455, 0x110251]
 
 
 @pytest.mark.parametrize("nrhs", [1, 12])
@@ -107,7 +109,11 @@ def test_dslash_policies(chip, orc, nrhs):
     with pytest.raises(chip.CvchipError):
         ctx.set_dslash_policy(0x321)      # y-block 3 does not divide LY = 8
     with pytest.raises(chip.CvchipError):
-        ctx.set_dslash_policy(0x50)        # traversal mode 5 does not exist
+        ctx.set_dslash_policy(0x50)        # column z extent 8 does not divide LZ = 6
+    with pytest.raises(chip.CvchipError):
+        ctx.set_dslash_policy(0x60)        # traversal mode 6 does not exist
+    with pytest.raises(chip.CvchipError):
+        ctx.set_dslash_policy(0x110355)    # column y extent 3 is not a power of two
     with pytest.raises(chip.CvchipError):
         ctx.set_dslash_policy(0x350141)     # brick t extent 5 does not divide T = 16
 
