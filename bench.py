@@ -233,16 +233,42 @@ def main():
 
     cpu = None
     if not a.no_cpu:
+        # the oracle restatement on the host cores: Q_phi_tbc with the reference's OpenMP site
+        # distribution (Q_phi.c:190-206), then one core for comparison, then BiCGStab
+        # iterations of invert_Qtm_her (operator threaded, vector operations serial as in the
+        # reference) on one rhs, extrapolated to the 12-rhs FP64 solve's iteration count
         import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         L = oracle.Lattice(dims, kappa, mu)
         phi = fin.download(0, 1)[0]
+        L.set_threads(threads)
+        L.Q_phi(g, phi)
+        reps = 10
+        c0 = time.perf_counter()
+        for _ in range(reps):
+            L.Q_phi(g, phi)
+        ct = (time.perf_counter() - c0) / reps
+        cpu = {"value": round(960 * V / ct / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": f"{reps} single-rhs Q_phi_tbc applications at {'x'.join(map(str, dims))} on {threads} "
+                         f"threads ({ct:.3f} s each; oracle restatement, 960 B/site)"}
+        L.set_threads(1)
         c0 = time.perf_counter()
         L.Q_phi(g, phi)
-        ct = time.perf_counter() - c0
-        cpu = {"value": round(960 * V / ct / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"one single-rhs Q_phi_tbc application at {'x'.join(map(str, dims))} "
-                         f"({ct:.1f}s, oracle restatement, 960 B/site)"}
-        log(f"cpu baseline {cpu}")
+        ct1 = time.perf_counter() - c0
+        extra["cpu_dslash_1core_GBs"] = round(960 * V / ct1 / 1e9, 3)
+        L.set_threads(threads)
+        b1 = cvc_amd.point_source(V, 0, 0, 0)
+        its = []
+        for n in (1, 3):
+            c0 = time.perf_counter()
+            L.invert_Qtm_her(g, b1, 1e-20, n)
+            its.append(time.perf_counter() - c0)
+        per_it = (its[1] - its[0]) / 2
+        extra["cpu_solve_iteration_s"] = round(per_it, 3)
+        if extra.get("solve_f64_iters"):
+            extra["cpu_solve_f64_12rhs_extrapolated_s"] = round(per_it * extra["solve_f64_iters"] * N, 1)
+        log(f"cpu baseline {cpu}; one core {extra['cpu_dslash_1core_GBs']} GB/s; "
+            f"BiCGStab iteration {per_it:.2f} s ({threads} threads)")
 
     out = {
         "metric": "Dslash HBM GB/s; 12-RHS solve s at 32^3x64, mixed vs FP64 at resid 1e-10",

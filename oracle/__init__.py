@@ -32,6 +32,7 @@ class _Lat(C.Structure):
         ("bc", C.c_double * 4), ("phase", (C.c_double * 2) * 4),
         ("perm", (C.c_int * 24) * 16), ("sign", (C.c_int * 24) * 16),
         ("iup", C.POINTER(C.c_int)), ("idn", C.POINTER(C.c_int)),
+        ("nthreads", C.c_int),
     ]
 
 
@@ -177,6 +178,10 @@ class Lattice:
         lib().orc_contract_twopoint(C.byref(self._s), _dp(out), int(idsource), int(idsink), cp, pp, int(n_c), mom,
                                     (C.c_int * 4)(*src))
         return out[:, 0] + 1j * out[:, 1]
+
+    def set_threads(self, n):
+        """Threads of the operator loop (the reference's OpenMP Q_phi_tbc, Q_phi.c:190-206)."""
+        lib().orc_set_threads(C.byref(self._s), int(n))
 
     def plaquette(self, gauge):
         return lib().orc_plaquette(C.byref(self._s), _dp(gauge))

@@ -9,6 +9,7 @@
 #include "cvc_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -92,6 +93,7 @@ static long lex(const orc_lat *L, int t, int x, int y, int z) {
 int orc_lat_init(orc_lat *L, int T, int LX, int LY, int LZ,
                  double kappa, double mu, const double bc[4]) {
   memset(L, 0, sizeof(*L));
+  L->nthreads = 1;
   L->T = T; L->LX = LX; L->LY = LY; L->LZ = LZ;
   L->V = (long)T * LX * LY * LZ;
   L->kappa = kappa; L->mu = mu;
@@ -221,14 +223,31 @@ static void hop_site(const orc_lat *L, const double *gauge, double *out,
   else memcpy(out, xi, sizeof(xi));
 }
 
+void orc_set_threads(orc_lat *L, int n) { L->nthreads = n > 0 ? n : 1; }
+
+/* the site loop of Q_phi_tbc; with threads as the reference's OpenMP
+ * version (Q_phi.c:197-206), cyclic over sites */
+static void hop_all(const orc_lat *L, const double *gauge, double *xi, const double *phi, double mutm, int g5) {
+  const int nt = L->nthreads > 1 ? L->nthreads : 1;
+  if (nt == 1) {
+    for (long ix = 0; ix < L->V; ix++) hop_site(L, gauge, xi + 24*ix, phi, ix, mutm, g5);
+    return;
+  }
+#pragma omp parallel num_threads(nt)
+  {
+    const long k = omp_get_thread_num(), n = omp_get_num_threads();
+    for (long ix = k; ix < L->V; ix += n) hop_site(L, gauge, xi + 24*ix, phi, ix, mutm, g5);
+  }
+}
+
 void orc_Q_phi(const orc_lat *L, const double *gauge, double *xi,
                const double *phi, double mutm) {
-  for (long ix = 0; ix < L->V; ix++) hop_site(L, gauge, xi + 24*ix, phi, ix, mutm, 0);
+  hop_all(L, gauge, xi, phi, mutm, 0);
 }
 
 void orc_Qf5(const orc_lat *L, const double *gauge, double *xi,
              const double *phi, double mutm) {
-  for (long ix = 0; ix < L->V; ix++) hop_site(L, gauge, xi + 24*ix, phi, ix, mutm, 1);
+  hop_all(L, gauge, xi, phi, mutm, 1);
 }
 
 void orc_g5_phi(const orc_lat *L, double *phi) {

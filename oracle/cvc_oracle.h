@@ -38,11 +38,16 @@ typedef struct {
   int sign[16][24];      /* gamma_sign                                     */
   int *iup;              /* [V][4] forward neighbours                      */
   int *idn;              /* [V][4] backward neighbours                     */
+  int nthreads;          /* OpenMP threads of the operator loop (default 1) */
 } orc_lat;
 
 int  orc_lat_init(orc_lat *L, int T, int LX, int LY, int LZ,
                   double kappa, double mu, const double bc[4]);
 void orc_lat_free(orc_lat *L);
+/* threads of orc_Q_phi / orc_Qf5 (the reference's OpenMP Q_phi_tbc,
+ * Q_phi.c:190-206: thread k takes the sites k, k + n, k + 2n, ...); the
+ * per-site arithmetic, and so every result, does not depend on it. */
+void orc_set_threads(orc_lat *L, int n);
 void orc_gamma_tables(int perm[16][24], int sign[16][24]);
 
 /* xi = D(mutm) phi; Q_phi.c:1092-1242 (Q_phi_tbc = this with mutm = g_mu). */

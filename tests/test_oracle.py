@@ -259,3 +259,16 @@ def test_bicgstab_her_matches_dense_solve(orc, small_system):
     assert it > 0 and tr < 1e-20
     xz = x.reshape(-1)[0::2] + 1j * x.reshape(-1)[1::2]
     assert np.max(np.abs(xz - want)) < 1e-9 * np.max(np.abs(want))
+
+
+def test_threaded_operator_is_bitwise_serial(orc, chip):
+    """The operator loop with OpenMP threads (the reference's Q_phi_tbc distribution,
+    Q_phi.c:190-206) returns the serial loop's bits: each site is computed alone."""
+    dims = (8, 4, 6, 4)
+    g = chip.gauge_random(dims, 3, 0.3)
+    L = orc.Lattice(dims, 0.1632, 0.004, (1.0, 0.3, 0.0, 0.0))
+    phi = np.random.default_rng(4).standard_normal((L.V, 24))
+    serial = [L.Q_phi(g, phi), L.Qf5(g, phi, -0.004)]
+    L.set_threads(4)
+    assert np.array_equal(L.Q_phi(g, phi), serial[0])
+    assert np.array_equal(L.Qf5(g, phi, -0.004), serial[1])
