@@ -431,22 +431,25 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     if constexpr (DM == DOT_TS) {
       // <out, in> = sum conj(out) in ; <out, out>   (or <out, r2> when a partner is given)
       if (P.partner) r2.load(own, i);
-      R a = 0, b = 0, n = 0;
+      // products in FP64: squares of a small-scale rhs underflow FP32
+      double a = 0, b = 0, n = 0;
 #pragma unroll
       for (int c = 0; c < 12; c++) {
-        a += acc[c].x * own[c].x + acc[c].y * own[c].y;
-        b += acc[c].x * own[c].y - acc[c].y * own[c].x;
-        n += acc[c].x * acc[c].x + acc[c].y * acc[c].y;
+        const double ox = acc[c].x, oy = acc[c].y, qx = own[c].x, qy = own[c].y;
+        a += ox * qx + oy * qy;
+        b += ox * qy - oy * qx;
+        n += ox * ox + oy * oy;
       }
       dv[0] = a; dv[1] = b; dv[2] = n;
     } else if constexpr (DM == DOT_R2) {
       C w[12];
       r2.load(w, i);
-      R a = 0, b = 0;
+      double a = 0, b = 0;
 #pragma unroll
       for (int c = 0; c < 12; c++) {
-        a += w[c].x * acc[c].x + w[c].y * acc[c].y;
-        b += w[c].x * acc[c].y - w[c].y * acc[c].x;
+        const double ox = acc[c].x, oy = acc[c].y, qx = w[c].x, qy = w[c].y;
+        a += qx * ox + qy * oy;
+        b += qx * oy - qy * ox;
       }
       dv[0] = a; dv[1] = b;
     }

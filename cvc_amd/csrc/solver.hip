@@ -100,14 +100,16 @@ __global__ void __launch_bounds__(256) k_r1_update(IO r1, IO s, IO t, IO r2, con
     s.load(vs, i);
     t.load(vt, i);
     r2.load(v2, i);
-    R n = 0, a = 0, b = 0;
+    // products in FP64: squares of a small-scale rhs underflow FP32
+    double n = 0, a = 0, b = 0;
 #pragma unroll
     for (int c = 0; c < 12; c++) {
       C o = csub(vs[c], cmul(w, vt[c]));
       vs[c] = o;
-      n += o.x * o.x + o.y * o.y;
-      a += v2[c].x * o.x + v2[c].y * o.y;
-      b += v2[c].x * o.y - v2[c].y * o.x;
+      const double ox = o.x, oy = o.y, qx = v2[c].x, qy = v2[c].y;
+      n += ox * ox + oy * oy;
+      a += qx * ox + qy * oy;
+      b += qx * oy - qy * ox;
     }
     vstore(r1, vs, i, nt);
     dv[0] = n; dv[1] = a; dv[2] = b;

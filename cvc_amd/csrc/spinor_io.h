@@ -113,9 +113,9 @@ struct IOHalf {
 #pragma unroll
     for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(v[c].x), fabsf(v[c].y)));
     // a site whose largest component is below kTiny is stored as zero: its
-    // reciprocal scale would overflow FP32 (the far tail of a point-source
-    // Krylov vector reaches the denormal range on large lattices)
-    constexpr float kTiny = 1e-30f;
+    // reciprocal scale would overflow FP32 below ~3e-39 (the far tail of a
+    // point-source Krylov vector reaches the denormal range on large lattices)
+    constexpr float kTiny = 1e-37f;
     const bool keep = m >= kTiny;
     const float inv = keep ? 1.0f / m : 0.f;
     typedef unsigned u4v __attribute__((ext_vector_type(4)));
