@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libcvchip.so")
 PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 SOLVE_LAGGED, SOLVE_VERBOSE, SOLVE_EO = 1, 2, 4
 ERR_NULL, ERR_NOCONV = -2, -3
+ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE = -100, -101, -102, -103
 
 # every entry point declared in include/cvchip.h
 EXPORTS = [
