@@ -10,13 +10,16 @@ A step is one application of the twisted-mass operator to all 12
 right-hand sides (FP64, one launch, links shared between rhs).  The value is
 its algorithmic HBM throughput: per site 12 x (192 B in + 192 B out) + 576 B
 of links = 5184 B, divided by the average launch duration from HIP events on
-the library's stream.  Beside it the line carries the 12-rhs solve times to a
-relative residual of 1e-10 (squared 1e-20) in FP64 and mixed precision, the
-time of one whole source (120 propagators and the Pi_mu_nu(x) contraction),
-the single-rhs operator figures, and the oracle's CPU operator timed on a
-bounded sample (one rhs) on this host.
+the library's stream.  Beside it the line carries the single-rhs, FP32, FP16
+and 48-rhs operator figures, the 12-rhs solve times to a relative residual of
+1e-10 (squared 1e-20) in FP64 and mixed precision, unpreconditioned and
+even/odd, the 48-rhs batched solve of four source positions with their pion
+correlators, the time of one whole source (120 propagators and the
+Pi_mu_nu(x) contraction), and the oracle's CPU operator on the host's threads
+(a bounded sample: ten single-rhs applications), on one core, and one CPU
+BiCGStab iteration extrapolated to the 12-rhs solve.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-source] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-48] [--no-source] [--no-cpu]
        python bench.py --copy-calibration   (reference copy kernel for the HBM counters)
 """
 from __future__ import annotations
