@@ -134,6 +134,13 @@ def main():
         m1 = ctx.dslash_timed(f1o, f1i, max(10, a.steps))
         extra[f"dslash_1rhs_f64_r{recon}_ms"] = round(m1, 5)
         extra[f"dslash_1rhs_f64_r{recon}_GBs"] = round(bps * V / (m1 * 1e-3) / 1e9, 1)
+    # the N-rhs FP64 operator with 2-row links (third row rebuilt; exact for unitary links only)
+    ctx.set_link_recon(12)
+    for _ in range(3):
+        ctx.dslash(fout, fin)
+    m12 = ctx.dslash_timed(fout, fin, max(10, a.steps))
+    extra[f"dslash_{N}rhs_f64_r12_ms"] = round(m12, 5)
+    extra[f"dslash_{N}rhs_f64_r12_GBs"] = round((N * 384 + 384) * V / (m12 * 1e-3) / 1e9, 1)
     ctx.set_link_recon(18)
     del f1i, f1o
     for prec, name, sb, lb in ((cvc_amd.PREC_F32, "f32", 96, 192), (cvc_amd.PREC_F16, "f16", 52, 192)):
