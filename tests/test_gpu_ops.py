@@ -53,6 +53,13 @@ def test_dslash_multi_rhs(chip, orc, nrhs):
     got = fout.download()
     for r in range(nrhs):
         assert rel_err(got[r], L.Q_phi(g, phi[r])) < 1e-13
+    # FP64 with 2-row links (the third row rebuilt; the generated links are SU(3))
+    ctx.set_link_recon(12)
+    ctx.dslash(fout, fin)
+    got = fout.download()
+    ctx.set_link_recon(18)
+    for r in range(nrhs):
+        assert rel_err(got[r], L.Q_phi(g, phi[r])) < 1e-13
     # reduced precisions through conversion
     for prec, tol in [(chip.PREC_F32, 2e-6), (chip.PREC_F16, 2e-3)]:
         a, b, back = ctx.field(nrhs, prec), ctx.field(nrhs, prec), ctx.field(nrhs)
