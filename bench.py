@@ -19,7 +19,8 @@ Pi_mu_nu(x) contraction), and the oracle's CPU operator on the host's threads
 (a bounded sample: ten single-rhs applications), on one core, and one CPU
 BiCGStab iteration extrapolated to the 12-rhs solve.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-48] [--no-source] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-solves12] [--no-48] [--no-source]
+                       [--no-cpu]
        python bench.py --copy-calibration   (reference copy kernel for the HBM counters)
 """
 from __future__ import annotations
@@ -47,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-source", action="store_true",
                     help="skip the whole-source run (120 propagators, FP16+norm inner solves, contraction)")
+    ap.add_argument("--no-solves12", action="store_true",
+                    help="skip the six 12-rhs solves (for profiling the whole-source run alone)")
     ap.add_argument("--no-48", action="store_true",
                     help="skip the 48-rhs batched solve (12 spin-colour x 4 source positions)")
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
@@ -164,7 +167,7 @@ def main():
         del f48i, f48o
     log(f"operator variants: {extra}")
 
-    if not a.no_solve:
+    if not a.no_solve and not a.no_solves12:
         b = ctx.field(N)
         b.upload(np.stack([cvc_amd.point_source(V, 0, k // 3, k % 3) for k in range(N)]))
         # the reference's algorithm (unpreconditioned), then even/odd preconditioned
