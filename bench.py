@@ -54,6 +54,8 @@ def parse():
                     help="skip the 48-rhs batched solve (12 spin-colour x 4 source positions)")
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
     ap.add_argument("--delta16", type=float, default=0.1, help="reliable-update factor of the FP16 mixed solver")
+    ap.add_argument("--delta16-eo", type=float, default=0.02,
+                    help="reliable-update factor of the FP16 mixed solver on the even/odd system")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (the reference kernel for HBM counters)")
     return ap.parse_args()
@@ -179,7 +181,8 @@ def main():
             torch.cuda.synchronize()
             s0 = time.perf_counter()
             rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
-                                    delta=a.delta16 if prec == cvc_amd.PREC_F16 else a.delta, check=False,
+                                    delta=(a.delta16_eo if eo else a.delta16) if prec == cvc_amd.PREC_F16 else a.delta,
+                                    check=False,
                                     eo=eo)
             torch.cuda.synchronize()
             el = time.perf_counter() - s0
@@ -237,7 +240,7 @@ def main():
         for rep, key in ((0, "source_first_solve_contract_s"), (1, "source_solve_contract_s")):
             torch.cuda.synchronize()
             s0 = time.perf_counter()
-            conn, ct, st = ctx.cvc_source((0, 0, 0, 0), 1e-20, 4000, cvc_amd.PREC_F16, a.delta16)
+            conn, ct, st = ctx.cvc_source((0, 0, 0, 0), 1e-20, 4000, cvc_amd.PREC_F16, a.delta16_eo)
             el = time.perf_counter() - s0
             extra[key] = round(el, 3)
             log(f"whole source {rep} (120 solves FP16+norm inner + contraction): {el:.2f}s {st}")
