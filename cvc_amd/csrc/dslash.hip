@@ -79,7 +79,19 @@ __device__ __forceinline__ void project(C h[2][3], Get get) {
 // w_a = row . h_a, then the upper spins take w_a and the lower spins
 // -+g_b w_{GCOL[b]}.  base[k * stride] is the stored link entry k (9 entries,
 // or the first two rows, the third rebuilt as conj(row0 x row1)).
-template <typename R, int RECON, int MU, bool FWD>
+// third row of an SU(3) link from its first two: conj(row0 x row1)
+template <typename C>
+__device__ __forceinline__ void rebuild_row3(C u[9]) {
+  const C a = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
+  const C b = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
+  const C d = csub(cmul(u[0], u[4]), cmul(u[1], u[3]));
+  u[6] = mk<decltype(a.x)>(a.x, -a.y);
+  u[7] = mk<decltype(a.x)>(b.x, -b.y);
+  u[8] = mk<decltype(a.x)>(d.x, -d.y);
+}
+
+// PHASE false: the entries at base already carry the boundary phase
+template <typename R, int RECON, int MU, bool FWD, bool PHASE = true>
 __device__ __forceinline__ void link_apply(cplx<R> acc[12], const cplx<R> h[2][3],
                                            const cplx<R> *__restrict__ base, uint32_t stride, cplx<R> ph) {
   using C = cplx<R>;
@@ -90,12 +102,7 @@ __device__ __forceinline__ void link_apply(cplx<R> acc[12], const cplx<R> h[2][3
   if constexpr (RECON == 12) {
 #pragma unroll
     for (int k = 0; k < 6; k++) u[k] = base[(size_t)k * stride];
-    const C a = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
-    const C b = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
-    const C d = csub(cmul(u[0], u[4]), cmul(u[1], u[3]));
-    u[6] = mk<R>(a.x, -a.y);
-    u[7] = mk<R>(b.x, -b.y);
-    u[8] = mk<R>(d.x, -d.y);
+    rebuild_row3(u);
   }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -112,7 +119,8 @@ __device__ __forceinline__ void link_apply(cplx<R> acc[12], const cplx<R> h[2][3
         w1 = cfmac(w1, e, h[1][j]);
       }
     }
-    if constexpr (FWD) {
+    if constexpr (!PHASE) {
+    } else if constexpr (FWD) {
       w0 = cmul(ph, w0);
       w1 = cmul(ph, w1);
     } else {
@@ -205,19 +213,23 @@ __device__ __forceinline__ void hop_pair(cplx<typename IO::real> acc[12], const 
                                          int coord, int L, uint32_t stride) {
   using R = typename IO::real;
   constexpr int KL = RECON == 18 ? 9 : 6;
+  // staged whole with the phase applied (see k_dslash): read as full links, no phase multiply
+  constexpr bool FOLD = LDSL && RECON == 12 && sizeof(R) == 4;
+  constexpr int KS = FOLD ? 9 : KL;
+  constexpr int RL = FOLD ? 18 : RECON;
   cplx<R> h[2][3];
   const uint32_t sm = coord == 0 ? s + (uint32_t)(L - 1) * stride : s - stride;
   const uint32_t sp = coord == L - 1 ? s - (uint32_t)(L - 1) * stride : s + stride;
   // backward: (1 + gamma_mu) U_mu(x-mu)^dagger phi(x-mu)
   ld.template proj<MU, false>(h, EO ? sm >> 1 : sm, MU == 3);
   if constexpr (LDSL && CVC_HOP_PIN == 2) pin_regs(&h[0][0], 6);
-  if constexpr (LDSL) link_apply<R, RECON, MU, false>(acc, h, Ul + (2 * MU) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
+  if constexpr (LDSL) link_apply<R, RL, MU, false, !FOLD>(acc, h, Ul + (2 * MU) * KS * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
   else link_apply<R, RECON, MU, false>(acc, h, U + (size_t)MU * KL * P.V + sm, P.V, P.ph[MU]);
   if constexpr (LDSL && (CVC_HOP_PIN == 1 || CVC_HOP_PIN == 2)) pin_regs(acc, 12);
   // forward: (1 - gamma_mu) U_mu(x) phi(x+mu)
   ld.template proj<MU, true>(h, EO ? sp >> 1 : sp, MU == 3);
   if constexpr (LDSL && CVC_HOP_PIN == 2) pin_regs(&h[0][0], 6);
-  if constexpr (LDSL) link_apply<R, RECON, MU, true>(acc, h, Ul + (2 * MU + 1) * KL * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
+  if constexpr (LDSL) link_apply<R, RL, MU, true, !FOLD>(acc, h, Ul + (2 * MU + 1) * KS * LDS_SITES + sl, LDS_SITES, P.ph[MU]);
   else link_apply<R, RECON, MU, true>(acc, h, U + (size_t)MU * KL * P.V + s, P.V, P.ph[MU]);
   if constexpr (LDSL && CVC_HOP_PIN > 0) pin_regs(acc, 12);
 }
@@ -324,8 +336,38 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
   const bool active = slot_v(sl, vs);
   const uint32_t i = vs * (uint32_t)P.N + r;
   double dv[3] = {0., 0., 0.};
-  __shared__ C Ul[LDSL ? 8 * KL * LDS_SITES : 1];
-  if constexpr (LDSL) {
+  // FP32 / FP16 with 2-row links: the staged links are whole (third row rebuilt)
+  // and carry the boundary phase, both done once per site here instead of once
+  // per rhs lane in every hop (the multi-rhs kernels in these precisions are
+  // bound by their arithmetic)
+  constexpr bool FOLD = LDSL && RECON == 12 && sizeof(R) == 4;
+  constexpr int KS = FOLD ? 9 : KL;
+  __shared__ C Ul[LDSL ? 8 * KS * LDS_SITES : 1];
+  // the link of hop hopi (2 mu: backward, U_mu(x - mu); 2 mu + 1: forward) of lexicographic site
+  auto link_site = [&](uint32_t hopi, uint32_t site) -> uint32_t {
+    if (hopi & 1u) return site;
+    const int mu = (int)(hopi >> 1);
+    const uint32_t str = mu == 0 ? P.sT : mu == 1 ? P.sX : mu == 2 ? (uint32_t)P.LZ : 1u;
+    const uint32_t L = (uint32_t)(mu == 0 ? P.T : mu == 1 ? P.LX : mu == 2 ? P.LY : P.LZ);
+    const uint32_t crd = (site / str) % L;
+    return crd == 0 ? site + (L - 1) * str : site - str;
+  };
+  if constexpr (FOLD) {
+    for (uint32_t e = threadIdx.x; e < 8u * S; e += blockDim.x) {
+      const uint32_t hopi = e / S, sle = e - hopi * S;
+      uint32_t ve;
+      if (!slot_v(sle, ve)) continue;
+      const uint32_t ls = link_site(hopi, lex_of(ve));
+      const int mu = (int)(hopi >> 1);
+      C u[9];
+#pragma unroll
+      for (int k = 0; k < 6; k++) u[k] = U[((size_t)mu * 6 + k) * P.V + ls];
+      rebuild_row3(u);
+#pragma unroll
+      for (int k = 0; k < 9; k++) Ul[(hopi * 9 + k) * LDS_SITES + sle] = cmul(u[k], P.ph[mu]);
+    }
+    __syncthreads();
+  } else if constexpr (LDSL) {
     const uint32_t tot = 8u * KL * S;
     for (uint32_t e = threadIdx.x; e < tot; e += blockDim.x) {
       const uint32_t hopi = e / (KL * S), rem = e - hopi * (KL * S);
