@@ -74,6 +74,34 @@ __device__ __forceinline__ void project(C h[2][3], Get get) {
   for (int c = 0; c < 3; c++) h[1][c] = cadd(get(3 + c), cunit<g1>(get(3 * p1 + c)));
 }
 
+// FP16 storage: the projection straight from the stored halves with the site's
+// scale n, each output real one pair of mixed-precision FMAs (half operand,
+// FP32 scale and accumulator): h = n A + u n B for the unit u of cunit.
+__device__ __forceinline__ float half_lo(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
+}
+template <int U>
+__device__ __forceinline__ float2 half_pair(uint32_t A, uint32_t B, float n) {
+  const float ax = __builtin_fmaf(half_lo(A), n, 0.0f), ay = __builtin_fmaf(half_hi(A), n, 0.0f);
+  if constexpr (U == 0) return make_float2(__builtin_fmaf(half_lo(B), n, ax), __builtin_fmaf(half_hi(B), n, ay));
+  else if constexpr (U == 1) return make_float2(__builtin_fmaf(half_lo(B), -n, ax), __builtin_fmaf(half_hi(B), -n, ay));
+  else if constexpr (U == 2) return make_float2(__builtin_fmaf(half_hi(B), -n, ax), __builtin_fmaf(half_lo(B), n, ay));
+  else return make_float2(__builtin_fmaf(half_hi(B), n, ax), __builtin_fmaf(half_lo(B), -n, ay));
+}
+template <int MU, bool FWD>
+__device__ __forceinline__ void project_half(float2 h[2][3], const uint32_t w[12], float n) {
+  constexpr int p0 = GCOL[MU][0], p1 = GCOL[MU][1];
+  constexpr int g0 = FWD ? uneg(GVAL[MU][0]) : GVAL[MU][0];
+  constexpr int g1 = FWD ? uneg(GVAL[MU][1]) : GVAL[MU][1];
+#pragma unroll
+  for (int c = 0; c < 3; c++) h[0][c] = half_pair<g0>(w[c], w[3 * p0 + c], n);
+#pragma unroll
+  for (int c = 0; c < 3; c++) h[1][c] = half_pair<g1>(w[3 + c], w[3 * p1 + c], n);
+}
+
 // acc += (1 - gamma_MU) U ph psi (FWD) or (1 + gamma_MU) U^dagger conj(ph) psi
 // (!FWD), from the projected h: one colour row of the link at a time,
 // w_a = row . h_a, then the upper spins take w_a and the lower spins
@@ -170,6 +198,18 @@ struct NbLoader {
     const uint32_t idx = site * N + r;
     if constexpr (IO::K == 1) {
       project<MU, FWD>(h, [&](int c) { return in.load_c(c, idx); });
+    } else if constexpr (IO::K == 4) {
+      const uint4 *q = reinterpret_cast<const uint4 *>(in.p);
+      uint32_t w[12];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const uint4 v = q[(size_t)k * in.VN + idx];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+      }
+      project_half<MU, FWD>(h, w, in.nrm[idx]);
     } else {
       C v[12];
       in.load(v, idx);
