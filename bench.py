@@ -177,21 +177,25 @@ def main():
                                (False, cvc_amd.PREC_F16, "mixed_f16"), (True, cvc_amd.PREC_F64, "eo_f64"),
                                (True, cvc_amd.PREC_F32, "eo_mixed_f32"), (True, cvc_amd.PREC_F16, "eo_mixed_f16")):
             x = ctx.field(N)
-            x.zero()                      # initial guess 0
-            torch.cuda.synchronize()
-            s0 = time.perf_counter()
-            rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
-                                    delta=(a.delta16_eo if eo else a.delta16) if prec == cvc_amd.PREC_F16 else a.delta,
-                                    check=False,
-                                    eo=eo)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - s0
-            extra[f"solve_{name}_s"] = round(el, 3)
+            # each solve twice from the zero initial guess: the first call also fills the
+            # library's pool of workspaces, the second is the steady-state cost
+            # (solve_<name>_s) that the speed-ups compare
+            for rep in (0, 1):
+                x.zero()
+                torch.cuda.synchronize()
+                s0 = time.perf_counter()
+                rc, st = ctx.invert_her(x, b, 1e-20, 20000 if prec == cvc_amd.PREC_F64 else 4000, prec=prec,
+                                        delta=(a.delta16_eo if eo else a.delta16) if prec == cvc_amd.PREC_F16
+                                        else a.delta,
+                                        check=False, eo=eo)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - s0
+                extra[f"solve_{name}_{'first_' if rep == 0 else ''}s"] = round(el, 3)
+                log(f"solve {name} call {rep}: rc={rc} {el:.2f}s {st}")
             extra[f"solve_{name}_iters"] = st["iterations"]
             extra[f"solve_{name}_true_res2"] = st["true_res2_max"]
             extra[f"solve_{name}_reliable_updates"] = st["reliable_updates"]
             extra[f"solve_{name}_rc"] = rc
-            log(f"solve {name}: rc={rc} {el:.2f}s {st}")
             del x
         if extra.get("solve_mixed_f32_s"):
             extra["mixed_f32_speedup"] = round(extra["solve_f64_s"] / extra["solve_mixed_f32_s"], 3)
@@ -257,7 +261,14 @@ def main():
         # iterations of invert_Qtm_her (operator threaded, vector operations serial as in the
         # reference) on one rhs, extrapolated to the 12-rhs FP64 solve's iteration count
         import oracle
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        # every core this process may run on; the environment's OMP_NUM_THREADS caps it (on
+        # the GPU hosts it is the CPU share that goes with one GPU)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = min(omp, avail) if omp > 0 else avail
+        extra["cpu_host_nproc"] = os.cpu_count()
+        extra["cpu_host_affinity"] = avail
+        extra["cpu_omp_num_threads_env"] = omp or None
         L = oracle.Lattice(dims, kappa, mu)
         phi = fin.download(0, 1)[0]
         L.set_threads(threads)
@@ -306,7 +317,10 @@ def main():
                                f"(T,LX,LY,LZ), kappa={kappa}, mu={mu}, gauge exp(i 0.3 H) seed 4",
                    "lattice": list(dims), "nrhs": N, "link_recon": 18},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dims, N)},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dims, N),
+                     "traffic_from": ("committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload: "
+                                      + os.path.relpath(TRAFFIC_FILE, ROOT))
+                     if measured_traffic(dims, N) is not None else None},
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -87,7 +87,7 @@ def build(verbose: bool = False) -> str:
     # a stand-in reference program that links the shim (used by the drop-in test)
     drv = os.path.join(os.path.dirname(HERE), "tests", "dropin")
     if os.path.exists(os.path.join(drv, "driver.cc")):
-        _run_all([["g++", "-O2", "-std=c++17", os.path.join(drv, "driver.cc"), DROPIN_OBJ, f"-L{HERE}", "-lcvchip",
+        _run_all([["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(drv, "driver.cc"), DROPIN_OBJ, f"-L{HERE}", "-lcvchip",
                    "-Wl,-rpath,$ORIGIN/../../cvc_amd", "-o", os.path.join(drv, "driver")]])
     return LIB
 

@@ -385,10 +385,13 @@ __global__ void k_sc_restart(RhsState *st, int N) {
 }
 
 // the FP64 residual recompute runs over all rhs at once: give every active rhs
-// its reliable update in the same pass
+// its reliable update in the same pass.  An rhs whose own trigger did not fire
+// (upd 0) has already advanced its search direction p in k_xp_update; it is
+// marked 2 so that k_p_update leaves that p alone (only its r and r0rn are
+// replaced).
 __global__ void k_sc_upd_all(RhsState *st, int N) {
   const int r = threadIdx.x;
-  if (r < N && !st[r].done) st[r].upd = 1;
+  if (r < N && !st[r].done && !st[r].upd) st[r].upd = 2;
 }
 
 __global__ void k_sc_clear_upd(RhsState *st, int N) {
@@ -396,12 +399,13 @@ __global__ void k_sc_clear_upd(RhsState *st, int N) {
   if (r < N) st[r].upd = 0;
 }
 
-// mixed, rhs flagged for a reliable update: p = r + beta (p - omega p2)
+// mixed, rhs whose own trigger fired (upd 1): p = r + beta (p - omega p2)
+// with the replaced residual r (k_xp_update skipped their p)
 template <typename IO>
 __global__ void __launch_bounds__(256) k_p_update(IO p, IO r1, IO p2, const RhsState *__restrict__ st,
                                                   uint32_t VN, int N) {
   CVC_ELEMWISE_PROLOGUE
-  if (!act || !st[rhs].upd || st[rhs].done) return;
+  if (!act || st[rhs].upd != 1 || st[rhs].done) return;
   using R = typename IO::real;
   using C = cplx<R>;
   const C w = mk<R>((R)st[rhs].omega[0], (R)st[rhs].omega[1]);
@@ -793,7 +797,14 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
       nconv++;
       if (S.hst[r].iter > maxit) maxit = S.hst[r].iter;
     }
-  // fold in what the low-precision accumulator still holds (non-converged rhs)
+  // fold in what the low-precision accumulator still holds (non-converged rhs;
+  // converged ones were folded at their last reliable update)
+  if (nconv < N) {
+    hipLaunchKernelGGL(k_sc_upd_all, dim3(1), dim3(256), 0, c->stream, S.st, N);
+    hipLaunchKernelGGL((k_accum_zero<I64, IOPlain<float>>), dim3(S.nblk), dim3(S.B), 0, c->stream, ix, ixl, S.st,
+                       S.VN, N);
+    CVC_HIP(hipGetLastError());
+  }
   double tr = 0.;
   if (!her && !eo) {
     TRY(true_residual(S, x, b, t64, c->mu, &tr, nullptr));

@@ -21,10 +21,12 @@
 // Krylov trajectory differ from the sequential reference sums.  The solvers
 // keep the reference's lagged return iterate (invert_Qtm.c:199-211).
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include "cvc_dropin.h"
 #include "cvchip.h"
 
 namespace cvc {
@@ -52,7 +54,8 @@ cvchip_ctx *g_ctx = nullptr;
 int g_dims[4] = {0, 0, 0, 0};
 double g_bc[4] = {0, 0, 0, 0};
 const double *g_gauge_seen = nullptr;
-double g_gauge_sum = 0.;
+uint64_t g_gauge_hash = 0;
+bool g_gauge_stale = false;   // set by cvchip_gauge_changed()
 double g_kappa_seen = 0., g_mu_seen = 0.;
 
 void die(int rc, const char *where) {
@@ -60,14 +63,38 @@ void die(int rc, const char *where) {
   exit(1);
 }
 
-double gauge_checksum(const double *g, long n) {
-  double s = 0.;
-  for (long i = 0; i < n; i += 97) s += g[i] * (double)((i % 13) + 1);
-  return s;
+// Hash of every 64-bit word of the gauge field (four independent
+// multiply-xorshift lanes, folded at the end): any change of any link entry,
+// also a single bit, changes it with probability 1 - 2^-64.  In place edits of
+// g_gauge_field (a local gauge transformation as in apply_gt_gauge,
+// cvc_utils.c:2044-2068, or a one-link finite difference) are seen by the next
+// operator or solver call.
+uint64_t gauge_hash(const double *g, long n) {
+  const uint64_t K = 0x9e3779b97f4a7c15ULL;
+  uint64_t h[4] = {1, 2, 3, 4};
+  long i = 0;
+  for (; i + 4 <= n; i += 4) {
+    for (int l = 0; l < 4; l++) {
+      uint64_t w;
+      memcpy(&w, g + i + l, sizeof(w));
+      h[l] = (h[l] ^ w) * K;
+      h[l] ^= h[l] >> 29;
+    }
+  }
+  for (; i < n; i++) {
+    uint64_t w;
+    memcpy(&w, g + i, sizeof(w));
+    h[0] = (h[0] ^ w) * K;
+    h[0] ^= h[0] >> 29;
+  }
+  uint64_t r = (uint64_t)n;
+  for (int l = 0; l < 4; l++) r = (r ^ h[l]) * K;
+  return r ^ (r >> 31);
 }
 
 // Context for the current globals; (re)uploads the gauge field when the
-// pointer or a strided checksum of it changed since the last call.
+// pointer or the hash of its contents changed since the last call, or when the
+// program announced a change with cvchip_gauge_changed().
 cvchip_ctx *ctx() {
   using namespace cvc;
   const int dims[4] = {T, LX, LY, LZ};
@@ -90,17 +117,23 @@ cvchip_ctx *ctx() {
     g_mu_seen = g_mu;
   }
   const long n = 72L * VOLUME;
-  const double sum = g_gauge_field ? gauge_checksum(g_gauge_field, n) : 0.;
-  if (g_gauge_field && (g_gauge_field != g_gauge_seen || sum != g_gauge_sum)) {
+  const uint64_t hash = g_gauge_field ? gauge_hash(g_gauge_field, n) : 0;
+  if (g_gauge_field && (g_gauge_stale || g_gauge_field != g_gauge_seen || hash != g_gauge_hash)) {
     int rc = cvchip_set_gauge(g_ctx, g_gauge_field);
     if (rc) die(rc, "cvchip_set_gauge");
     g_gauge_seen = g_gauge_field;
-    g_gauge_sum = sum;
+    g_gauge_hash = hash;
+    g_gauge_stale = false;
   }
   return g_ctx;
 }
 
 }  // namespace
+
+// No counterpart in the reference (its operator reads g_gauge_field on every
+// call): a program that changed g_gauge_field in place calls this to have the
+// next operator or solver call upload it again, whatever the hash says.
+extern "C" void cvchip_gauge_changed(void) { g_gauge_stale = true; }
 
 namespace cvc {
 

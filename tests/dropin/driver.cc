@@ -5,7 +5,10 @@
 //   driver <dir> T LX LY LZ kappa mu
 //     reads  <dir>/gauge.bin (V*72), <dir>/phi.bin (V*24)
 //     writes <dir>/qphi.bin (Q_phi_tbc phi), <dir>/qf5.bin (Qf5(phi, -mu)),
-//            <dir>/x.bin (invert_Qtm_her solution of D x = phi), <dir>/ret.txt
+//            <dir>/x.bin (invert_Qtm_her solution of D x = phi), <dir>/ret.txt,
+//            <dir>/qphi_edit.bin (Q_phi_tbc phi after one link entry was changed
+//            in place, no notice), <dir>/qphi_edit2.bin (after a second in-place
+//            change announced by cvchip_gauge_changed())
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -28,6 +31,7 @@ void Qf5(double *xi, double *phi, double mutm);
 void spinor_scalar_product_co(complex *w, double *xi, double *phi, int V);
 int invert_Qtm_her(double *xi, double *phi, int kwork);
 }  // namespace cvc
+#include "cvc_dropin.h"
 
 static std::vector<double> readf(const char *dir, const char *name, size_t n) {
   char path[1024];
@@ -72,6 +76,15 @@ int main(int argc, char **argv) {
   spinor_scalar_product_co(&w, phi.data(), out.data(), VOLUME);
   int ret = invert_Qtm_her(x.data(), phi.data(), 0);
   writef(dir, "x.bin", x.data(), x.size());
+  // in-place edits of one link entry, U_1(site 5)[0][0] (real part), as a
+  // one-link finite difference would make; the second one is announced
+  g_gauge_field[72 * 5 + 18 * 1 + 0] += 1e-3;
+  Q_phi_tbc(out.data(), phi.data());
+  writef(dir, "qphi_edit.bin", out.data(), out.size());
+  g_gauge_field[72 * 5 + 18 * 1 + 1] -= 1e-3;
+  cvchip_gauge_changed();
+  Q_phi_tbc(out.data(), phi.data());
+  writef(dir, "qphi_edit2.bin", out.data(), out.size());
   // a missing workspace field gives -2 as in the reference
   g_spinor_field[3] = NULL;
   int ret2 = invert_Qtm_her(x.data(), phi.data(), 0);

@@ -53,3 +53,23 @@ def test_stand_in_program(tmp_path, chip, orc):
     # lagged iterate (reference semantics): within a factor of the tolerance 1e-20
     assert L.dot_re(res, res) / L.dot_re(phi, phi) < 1e-18
     assert "BiCGStab converged" in r.stdout
+    # in-place edits of g_gauge_field between calls reach the device gauge field
+    g1 = g.copy().reshape(-1)
+    g1[72 * 5 + 18 * 1 + 0] += 1e-3
+    e1 = np.fromfile(tmp_path / "qphi_edit.bin").reshape(L.V, 24)
+    assert rel_err(e1, L.Q_phi(g1.reshape(g.shape), phi)) < 1e-13
+    assert rel_err(e1, qphi) > 1e-8
+    g1[72 * 5 + 18 * 1 + 1] -= 1e-3
+    e2 = np.fromfile(tmp_path / "qphi_edit2.bin").reshape(L.V, 24)
+    assert rel_err(e2, L.Q_phi(g1.reshape(g.shape), phi)) < 1e-13
+
+
+def test_shim_exports_its_header():
+    """Every function include/cvc_dropin.h declares is defined by the shim object."""
+    import re
+    text = open(os.path.join(ROOT, "include", "cvc_dropin.h")).read()
+    decl = set(re.findall(r"^\s*\w+\s*\*?\s*(cvchip_\w+)\s*\(", text, re.M))
+    assert decl == {"cvchip_gauge_changed"}
+    out = subprocess.run(["nm", OBJ], capture_output=True, text=True, check=True).stdout
+    defined = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert decl <= defined
