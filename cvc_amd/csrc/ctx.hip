@@ -441,13 +441,21 @@ extern "C" int cvchip_ctx_set_dslash_policy(cvchip_ctx *c, int policy) {
   if (!c) return CVCHIP_ERR_NULL;
   const int mode = (policy >> 4) & 15, yb = (policy >> 8) & 255;
   const int bt = (policy >> 16) & 15, bx = (policy >> 20) & 15;
-  if (policy < 0 || policy >= (1 << 28) || mode > 5) {
-    set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..5, 8-15 y-block, "
-              "16-19 brick t (traversal 5: log2 of the column's z extent), 20-23 brick x, 24-27 LDS padding",
+  if (policy < 0 || policy >= (1 << 28) || mode > 6) {
+    set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..6, 8-15 y-block "
+              "(traversal 6: march length), 16-19 brick t (traversal 5: log2 of the column's z extent; 6: patch x), "
+              "20-23 brick x (traversal 6: patch y), 24-27 LDS padding",
               policy);
     return CVCHIP_ERR_ARG;
   }
-  if (mode == 5) {
+  if (mode == 6) {
+    // t march: patch extents divide LX and LY
+    if ((bt && c->lat.LX % bt) || (bx && c->lat.LY % bx)) {
+      set_error("operator policy 0x%x: t-march patch extents x %d, y %d must divide LX=%d, LY=%d", policy, bt, bx,
+                c->lat.LX, c->lat.LY);
+      return CVCHIP_ERR_ARG;
+    }
+  } else if (mode == 5) {
     // column extents: powers of two dividing the lattice
     const int zb = bt ? 1 << bt : 8;
     const int cy = yb ? yb : 4, cx = bx ? bx : 4;

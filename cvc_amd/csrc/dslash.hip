@@ -43,6 +43,11 @@ struct DParams {
   uint32_t yb, runlen, bpr, nrun;
   uint32_t bt, bx, ncx, bpb, bpcol;
   uint32_t zb, ncy, ncz, lzb, lyz, lA, colsz, sTt, sXt;
+  // Mode 6 (k_dslash_march): mS columns per workgroup marched through mM
+  // slices; patches of mXP x mYP x LZ columns (mNPX along x), mCP chunks of mS
+  // columns per patch, mNR = T / mM slice ranges; mXcd: patch p on XCD p mod 8.
+  uint32_t mS, mM, mXP, mYP, mCP, mNR, mNPX;
+  int mXcd;
   int nt;   // streaming output stores
   // Even/odd operator (EO kernels): the traversal runs over the half lattice
   // (T, LX, LY, LZ/2); half-site h of parity par is the lexicographic site
@@ -540,6 +545,273 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
   else if constexpr (DM == DOT_R2) block_reduce_store<2>(dv, part, P.N);
 }
 
+// ------------------------------------------------------- t-marching operator
+//
+// Multi-rhs operator on the whole lattice in which each workgroup owns S
+// columns (sites of one (x, y, z) cross-section, all N rhs) and walks them
+// through M consecutive time slices.  The backward t hop of slice t+1,
+// (1 + gamma_0) U_0(t)^dagger phi(t), is formed from the own spinor phi(t)
+// while slice t is processed and carried to the next step in registers as the
+// six complex numbers it depends on, so the t neighbours are not read again
+// from memory: per step a thread reads phi(t) (its own, which the same thread
+// read one step before as the forward neighbour, so it comes from L2), the six
+// spatial neighbours and phi(t+1).  Columns are grouped in patches of
+// XP x YP x LZ sites; XCD j takes the patches p = j mod 8, and the workgroups
+// of one XCD run the chunks of one patch over one range of M slices side by
+// side, so the spatial neighbours of a chunk are read by neighbouring
+// workgroups of the same XCD at about the same step.  The links of a step
+// (U_0(t), U_k(t), U_k(t - k) for k = x, y, z) are staged in LDS once per
+// workgroup and shared by the N rhs lanes of a column.
+//
+// Accumulation: acc starts at -2 (1/(2 kappa) + i mutm gamma5) phi(t), the
+// eight hops are added, and the output is -acc/2 (the factors -2 and -1/2 are
+// exact), i.e. phi/(2 kappa) + i mutm gamma5 phi - hop/2 as in Q_phi.c:58-174
+// with a different rounding order.
+
+// w_a = link row . h_a for the three rows (w[a][i]), the boundary phase
+// applied (PHASE) as in link_apply; expand_w adds the hop's four spin rows.
+template <typename R, int RECON, bool FWD, bool PHASE>
+__device__ __forceinline__ void link_mul(cplx<R> w[2][3], const cplx<R> h[2][3], const cplx<R> *__restrict__ base,
+                                         uint32_t stride, cplx<R> ph) {
+  using C = cplx<R>;
+  C u[9];
+  if constexpr (RECON == 12) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) u[k] = base[(size_t)k * stride];
+    rebuild_row3(u);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    C w0 = mk<R>(0, 0), w1 = mk<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int k = FWD ? 3 * i + j : 3 * j + i;
+      const C e = RECON == 18 ? base[(size_t)k * stride] : u[k];
+      if constexpr (FWD) {
+        w0 = cfma(w0, e, h[0][j]);
+        w1 = cfma(w1, e, h[1][j]);
+      } else {
+        w0 = cfmac(w0, e, h[0][j]);
+        w1 = cfmac(w1, e, h[1][j]);
+      }
+    }
+    if constexpr (!PHASE) {
+    } else if constexpr (FWD) {
+      w0 = cmul(ph, w0);
+      w1 = cmul(ph, w1);
+    } else {
+      w0 = cmulc(ph, w0);
+      w1 = cmulc(ph, w1);
+    }
+    w[0][i] = w0;
+    w[1][i] = w1;
+  }
+}
+
+template <typename C, int MU, bool FWD>
+__device__ __forceinline__ void expand_w(C acc[12], const C w[2][3]) {
+  constexpr int q2 = GCOL[MU][2], q3 = GCOL[MU][3];
+  constexpr int l2 = FWD ? uneg(GVAL[MU][2]) : GVAL[MU][2];
+  constexpr int l3 = FWD ? uneg(GVAL[MU][3]) : GVAL[MU][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    acc[i] = cadd(acc[i], w[0][i]);
+    acc[3 + i] = cadd(acc[3 + i], w[1][i]);
+    acc[6 + i] = cadd(acc[6 + i], cunit<l2>(q2 == 0 ? w[0][i] : w[1][i]));
+    acc[9 + i] = cadd(acc[9 + i], cunit<l3>(q3 == 0 ? w[0][i] : w[1][i]));
+  }
+}
+
+// the links of one step, staged per column: slot 0 U_0(t), slots 1-3 U_k(t),
+// slots 4-6 U_k(t - k)
+constexpr int kMarchSlots = 7;
+
+#ifndef CVC_MARCH_WAVES64
+#define CVC_MARCH_WAVES64 3
+#endif
+#ifndef CVC_MARCH_WAVES32
+#define CVC_MARCH_WAVES32 4
+#endif
+template <typename R> struct MarchWaves {
+  static constexpr int value = sizeof(R) == 8 ? CVC_MARCH_WAVES64 : CVC_MARCH_WAVES32;
+};
+
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM>
+__global__ void __launch_bounds__(256, MarchWaves<typename IOI::real>::value)
+k_dslash_march(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParams<typename IOI::real> P,
+               IOI r2, double *__restrict__ part) {
+  using R = typename IOI::real;
+  using C = cplx<R>;
+  constexpr int KL = RECON == 18 ? 9 : 6;
+  // FP32 with 2-row links: staged whole with the phase applied (as in k_dslash)
+  constexpr bool FOLD = RECON == 12 && sizeof(R) == 4;
+  constexpr int KS = FOLD ? 9 : KL;
+  constexpr int RL = FOLD ? 18 : RECON;
+  const uint32_t S = P.mS;
+  // workgroup -> (patch, range of slices, chunk of the patch's columns)
+  uint32_t k = blockIdx.x, xcd = 0;
+  if (P.mXcd) {
+    xcd = blockIdx.x % kXcds;
+    k = blockIdx.x / kXcds;
+  }
+  const uint32_t ch = k % P.mCP, q = k / P.mCP;
+  const uint32_t rng = q % P.mNR, pl = q / P.mNR;
+  const uint32_t patch = P.mXcd ? pl * kXcds + xcd : pl;
+  const uint32_t px = patch % P.mNPX, py = patch / P.mNPX;
+  const uint32_t r = threadIdx.x % (uint32_t)P.N, cl = threadIdx.x / (uint32_t)P.N;
+  // column cl of the chunk: z fastest, then y, then x inside the patch
+  auto column = [&](uint32_t c, uint32_t &sxyz) -> bool {
+    const uint32_t j = ch * S + c;
+    if (c >= S || j >= P.mXP * P.mYP * (uint32_t)P.LZ) return false;
+    const uint32_t z = j % (uint32_t)P.LZ, j2 = j / (uint32_t)P.LZ;
+    const uint32_t y = py * P.mYP + j2 % P.mYP, x = px * P.mXP + j2 / P.mYP;
+    sxyz = (x * (uint32_t)P.LY + y) * (uint32_t)P.LZ + z;
+    return true;
+  };
+  uint32_t sxyz = 0;
+  const bool active = column(cl, sxyz);
+  uint32_t cx = 0, cy = 0, cz = 0;
+  if (active) {
+    cz = sxyz % (uint32_t)P.LZ;
+    cy = (sxyz / (uint32_t)P.LZ) % (uint32_t)P.LY;
+    cx = sxyz / P.sX;
+  }
+  const uint32_t t0 = rng * P.mM;
+  extern __shared__ unsigned char dyn_lds[];
+  C *Ul = reinterpret_cast<C *>(dyn_lds);   // [kMarchSlots][KS][S]
+  const NbLoader<IOI, false> ld{in, nullptr, (uint32_t)P.N, r, 0, 0};
+  double dv[3] = {0., 0., 0.};
+
+  // carry: (U_0^dagger (1 + gamma_0) phi)(t - 1) as the six complex numbers w
+  C carry[2][3];
+  if (active) {
+    const uint32_t tm = t0 == 0 ? (uint32_t)P.T - 1 : t0 - 1;
+    const uint32_t sm = tm * P.sT + sxyz;
+    C h[2][3];
+    ld.template proj<0, false>(h, sm, false);
+    link_mul<R, RECON, false, true>(carry, h, U + sm, P.V, P.ph[0]);
+  }
+
+  for (uint32_t step = 0; step < P.mM; step++) {
+    const uint32_t t = t0 + step;
+    __syncthreads();   // the previous step's links are no longer read
+    for (uint32_t e = threadIdx.x; e < (uint32_t)kMarchSlots * S; e += blockDim.x) {
+      const uint32_t slot = e / S, c = e - slot * S;
+      uint32_t cs;
+      if (!column(c, cs)) continue;
+      const int mu = slot < 4 ? (int)slot : (int)slot - 3;
+      uint32_t ls = t * P.sT + cs;
+      if (slot >= 4) {   // U_k(t - k)
+        const uint32_t str = mu == 1 ? P.sX : mu == 2 ? (uint32_t)P.LZ : 1u;
+        const uint32_t L = (uint32_t)(mu == 1 ? P.LX : mu == 2 ? P.LY : P.LZ);
+        const uint32_t crd = (cs / str) % L;
+        ls = crd == 0 ? ls + (L - 1) * str : ls - str;
+      }
+      if constexpr (FOLD) {
+        C u[9];
+#pragma unroll
+        for (int kk = 0; kk < 6; kk++) u[kk] = U[((size_t)mu * 6 + kk) * P.V + ls];
+        rebuild_row3(u);
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) Ul[(slot * 9 + kk) * S + c] = cmul(u[kk], P.ph[mu]);
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < KL; kk++) Ul[(slot * KL + kk) * S + c] = U[((size_t)mu * KL + kk) * P.V + ls];
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    const uint32_t s = t * P.sT + sxyz;
+    const uint32_t i = s * (uint32_t)P.N + r;
+    C acc[12];
+    {
+      C own[12];
+      ld.full(own, s, false);
+      // acc = -2 (1/(2 kappa) + i mutm gamma5) phi   (gamma5 = +1 on spins 0,1, -1 on 2,3)
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        const R m = c < 6 ? P.mutm : -P.mutm;
+        acc[c].x = (R)-2 * (P.k2 * own[c].x - m * own[c].y);
+        acc[c].y = (R)-2 * (P.k2 * own[c].y + m * own[c].x);
+      }
+      // backward t hop from the carry, then the carry for slice t+1 from phi(t)
+      expand_w<C, 0, false>(acc, carry);
+      C h[2][3];
+      project<0, false>(h, [&](int c) { return own[c]; });
+      link_mul<R, RL, false, !FOLD>(carry, h, Ul + cl, S, P.ph[0]);
+    }
+    pin_regs(acc, 12);
+    pin_regs(&carry[0][0], 6);
+    // spatial hops
+#define CVC_MHOP(MU, CRD, L, STR)                                                                              \
+    {                                                                                                          \
+      const uint32_t sm_ = (CRD) == 0 ? s + (uint32_t)((L) - 1) * (STR) : s - (STR);                          \
+      const uint32_t sp_ = (CRD) == (uint32_t)(L) - 1 ? s - (uint32_t)((L) - 1) * (STR) : s + (STR);           \
+      C h[2][3], w[2][3];                                                                                      \
+      ld.template proj<MU, false>(h, sm_, false);                                                              \
+      pin_regs(&h[0][0], 6);                                                                                   \
+      link_mul<R, RL, false, !FOLD>(w, h, Ul + (3 + MU) * KS * S + cl, S, P.ph[MU]);                           \
+      expand_w<C, MU, false>(acc, w);                                                                          \
+      pin_regs(acc, 12);                                                                                       \
+      ld.template proj<MU, true>(h, sp_, false);                                                               \
+      pin_regs(&h[0][0], 6);                                                                                   \
+      link_mul<R, RL, true, !FOLD>(w, h, Ul + (MU)*KS * S + cl, S, P.ph[MU]);                                  \
+      expand_w<C, MU, true>(acc, w);                                                                           \
+      pin_regs(acc, 12);                                                                                       \
+    }
+    CVC_MHOP(1, cx, P.LX, P.sX);
+    CVC_MHOP(2, cy, P.LY, (uint32_t)P.LZ);
+    CVC_MHOP(3, cz, P.LZ, 1u);
+#undef CVC_MHOP
+    // forward t hop: (1 - gamma_0) U_0(t) phi(t+1), last, so that phi(t+1) is
+    // still in L2 when the next step reads it as its own spinor
+    {
+      const uint32_t sp = t == (uint32_t)P.T - 1 ? sxyz : s + P.sT;
+      C h[2][3], w[2][3];
+      ld.template proj<0, true>(h, sp, false);
+      pin_regs(&h[0][0], 6);
+      link_mul<R, RL, true, !FOLD>(w, h, Ul + cl, S, P.ph[0]);
+      expand_w<C, 0, true>(acc, w);
+    }
+    const R mh = (R)-0.5;
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      acc[c].x *= mh;
+      acc[c].y *= mh;
+      if (G5OUT && c >= 6) { acc[c].x = -acc[c].x; acc[c].y = -acc[c].y; }
+    }
+    if (P.nt) out.store_nt(acc, i);
+    else out.store(acc, i);
+    if constexpr (DM == DOT_TS) {
+      C own[12];
+      if (P.partner) r2.load(own, i);
+      else in.load(own, i);
+      double a = 0, b = 0, n = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        const double ox = acc[c].x, oy = acc[c].y, qx = own[c].x, qy = own[c].y;
+        a += ox * qx + oy * qy;
+        b += ox * qy - oy * qx;
+        n += ox * ox + oy * oy;
+      }
+      dv[0] += a; dv[1] += b; dv[2] += n;
+    } else if constexpr (DM == DOT_R2) {
+      C w[12];
+      r2.load(w, i);
+      double a = 0, b = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        const double ox = acc[c].x, oy = acc[c].y, qx = w[c].x, qy = w[c].y;
+        a += qx * ox + qy * oy;
+        b += qx * oy - qy * ox;
+      }
+      dv[0] += a; dv[1] += b;
+    }
+  }
+  if constexpr (DM == DOT_TS) block_reduce_store<3>(dv, part, P.N);
+  else if constexpr (DM == DOT_R2) block_reduce_store<2>(dv, part, P.N);
+}
+
 // ------------------------------------------------------------- launcher
 
 template <typename R>
@@ -577,7 +849,42 @@ struct Traversal {
   uint32_t yb, runlen, bpr, nrun;
   uint32_t bt, bx, ncx, bpb, bpcol;
   uint32_t zb, ncy, ncz, lzb, lyz, lA, colsz;
+  uint32_t mS, mM, mXP, mYP, mCP, mNR, mNPX;
+  int mXcd;
+  unsigned threads;   // workgroup size
 };
+
+// t-march geometry (mode 6) for N rhs on the whole lattice L; bits 8-15 of the
+// policy give the march length M (0: 16; the largest divisor of T not above
+// it is used), bits 16-19 and 20-23 the patch's x and y extents (0: 8; a
+// value that does not divide LX or LY falls back to 1).
+static bool march_geometry(const cvchip_ctx *c, int N, Traversal &t) {
+  const Lattice &L = c->lat;
+  if (N < 1 || N > 192) return false;
+  uint32_t S = 192u / (uint32_t)N;
+  if (S > 48) S = 48;
+  uint32_t M = (uint32_t)((c->dslash_policy >> 8) & 255);
+  if (M == 0) M = 16;
+  while (M > 1 && L.T % (int)M) M--;
+  uint32_t XP = (uint32_t)((c->dslash_policy >> 16) & 15), YP = (uint32_t)((c->dslash_policy >> 20) & 15);
+  if (XP == 0) XP = 8;
+  if (YP == 0) YP = 8;
+  if (L.LX % (int)XP) XP = 1;
+  if (L.LY % (int)YP) YP = 1;
+  t.mode = 6;
+  t.mS = S;
+  t.mM = M;
+  t.mXP = XP;
+  t.mYP = YP;
+  t.mNPX = (uint32_t)L.LX / XP;
+  const uint32_t npatch = t.mNPX * ((uint32_t)L.LY / YP);
+  t.mCP = (XP * YP * (uint32_t)L.LZ + S - 1) / S;
+  t.mNR = (uint32_t)L.T / M;
+  t.mXcd = npatch % kXcds == 0 ? 1 : 0;
+  t.nblk = npatch * t.mNR * t.mCP;
+  t.threads = S * (uint32_t)N;
+  return true;
+}
 
 static int log2_exact(uint32_t v) {   // -1 unless v is a power of two
   if (v == 0 || (v & (v - 1))) return -1;
@@ -600,6 +907,11 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
   const uint32_t V = (uint32_t)L.V, slice = (uint32_t)(L.LX * L.LY * L.LZ);
   const uint32_t S = (uint32_t)(block_for(N) / N);
   int want = (c->dslash_policy >> 4) & 15;
+  if (want == 6) {
+    Traversal t{};
+    if (!eo && march_geometry(c, N, t)) return t;
+    want = 4;   // the even/odd operator does not march
+  }
   uint32_t yb = (uint32_t)((c->dslash_policy >> 8) & 255);
   // automatic: bricks of 4 slices x 4 x-planes x yb y-rows for the multi-rhs
   // operator, else one slice per XCD in y blocks (both measured at 64x32^3
@@ -620,6 +932,7 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
     }
   }
   Traversal t{};
+  t.threads = (unsigned)block_for(N);
   if (want == 5) {
     const uint32_t cyb = (c->dslash_policy >> 8) & 255 ? yb : 4u;
     const uint32_t lz = (c->dslash_policy >> 16) & 15;
@@ -730,6 +1043,14 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   P.colsz = tr.colsz;
   P.sXt = (uint32_t)c->lat.LY * P.LZt;
   P.sTt = (uint32_t)c->lat.LX * P.sXt;
+  P.mS = tr.mS;
+  P.mM = tr.mM;
+  P.mXP = tr.mXP;
+  P.mYP = tr.mYP;
+  P.mCP = tr.mCP;
+  P.mNR = tr.mNR;
+  P.mNPX = tr.mNPX;
+  P.mXcd = tr.mXcd;
   // streaming stores only for outputs that cannot stay in the 256 MB MALL
   // (bit 3: for every output)
   P.nt = (((c->dslash_policy >> 2) & 1) && out.bytes >= ((size_t)256 << 20)) || ((c->dslash_policy >> 3) & 1);
@@ -747,6 +1068,28 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   IOI rr = r2 ? make_io<IOI>(*r2) : ii;
   IOI ow = (eo && eo->own) ? make_io<IOI>(*eo->own) : ii;
   dim3 g(nblk), b(B);
+  if (!EO && tr.mode == 6) {
+    using CR = cplx<R>;
+    constexpr int KSM = (RECON == 12 && sizeof(R) == 4) ? 9 : (RECON == 18 ? 9 : 6);
+    // bits 24-27: extra LDS per block in 8 KiB units (caps the blocks per CU)
+    const size_t lds = (size_t)kMarchSlots * KSM * tr.mS * sizeof(CR) + (size_t)((c->dslash_policy >> 24) & 15) * 8192;
+    dim3 bm(tr.threads);
+#define CVC_MLAUNCH(G5, DMV) \
+    hipLaunchKernelGGL((k_dslash_march<IOI, IOO, RECON, G5, DMV>), g, bm, lds, c->stream, o, ii, U, P, rr, part)
+    if (g5) {
+      if (dm == DOT_NONE) CVC_MLAUNCH(true, DOT_NONE);
+      else if (dm == DOT_TS) CVC_MLAUNCH(true, DOT_TS);
+      else CVC_MLAUNCH(true, DOT_R2);
+    } else {
+      if (dm == DOT_NONE) CVC_MLAUNCH(false, DOT_NONE);
+      else if (dm == DOT_TS) CVC_MLAUNCH(false, DOT_TS);
+      else CVC_MLAUNCH(false, DOT_R2);
+    }
+#undef CVC_MLAUNCH
+    CVC_HIP(hipGetLastError());
+    if (nblk_out) *nblk_out = (int)nblk;
+    return 0;
+  }
   const bool multi = B / N <= LDS_SITES;
   const bool ldsl = multi && (c->dslash_policy & 1);
   const bool ldss = !EO && multi && (c->dslash_policy & 2) && tr.mode != 5;

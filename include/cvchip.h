@@ -82,7 +82,11 @@ int cvchip_ctx_set_solver_flags(cvchip_ctx *ctx, unsigned flags);
  * block-to-site traversal: 0 automatic, 1 time slices cut into 8 XCD chunks,
  * 2 one time slice per XCD walked in y blocks, 3 linear, 4 bricks of bt x bx x
  * yb x LZ sites marched along t per XCD, 5 columns of bx x yb x zb sites
- * (powers of two) swept along t per XCD.  Bits 8-15: the y-block height of
+ * (powers of two) swept along t per XCD, 6 t march (multi-rhs operator on the
+ * whole lattice: each workgroup walks its columns through consecutive time
+ * slices and carries the backward t hop in registers; bits 8-15 the march
+ * length (0: 16), bits 16-19 and 20-23 the x and y extents of the patches of
+ * columns that one XCD works on together (0: 8)).  Bits 8-15: the y-block height of
  * traversals 2, 4 and 5 (0: the largest of 16, 8, 4, 2 dividing LY; for the
  * automatic bricks the largest of 8, 4, 2 with yb * N <= 96; 4 for traversal
  * 5); bits 16-19 bt of traversal 4 (0: 4) or log2 zb of traversal 5 (0: zb 8)

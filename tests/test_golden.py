@@ -58,3 +58,33 @@ def test_oracle_contraction_is_frozen(gold, orc):
     conn, ct = L.cvc_contract(g, point_props(L, g, src, 1e-26), src)
     assert np.array_equal(conn, gold["conn4"])
     assert np.array_equal(ct, gold["contact4"])
+
+
+def test_e2e_fixture_is_frozen(orc):
+    """golden_e2e.npz (tests/golden/make_golden_e2e.py): the 4^4 source is recomputed
+    bit for bit from the oracle's 120 solves; the 8^4 source (its solves take minutes
+    here) satisfies the Ward identity with its own contact term and was solved to the
+    recorded tolerance."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_golden_e2e import point_props
+    import cvc_amd
+    e2e = np.load(os.path.join(HERE, "golden", "golden_e2e.npz"), allow_pickle=False)
+    assert float(e2e["sp"]) == 1e-28
+    dims = tuple(int(v) for v in e2e["dims4"])
+    src = tuple(int(v) for v in e2e["src4"])
+    g = cvc_amd.gauge_random(dims, 1, 0.3)
+    L = orc.Lattice(dims, 0.163, 0.01, BC)
+    conn, ct = L.cvc_contract(g, point_props(dims, g, src, 1e-28, 1), src)
+    assert np.array_equal(conn, e2e["conn4"]) and np.array_equal(ct, e2e["contact4"])
+    dims8 = tuple(int(v) for v in e2e["dims8"])
+    assert dims8 == (8, 8, 8, 8) and tuple(int(v) for v in e2e["src8"]) == (0, 0, 0, 0)
+    L8 = orc.Lattice(dims8, 0.163, 0.01, BC)
+    conn8, ct8 = e2e["conn8"], e2e["contact8"]
+    w = L8.wi(conn8)
+    w = w[..., 0] + 1j * w[..., 1]
+    want = np.zeros_like(w)
+    want[0, :] += ct8
+    for nu in range(4):
+        want[8 ** (3 - nu), nu] -= ct8[nu]
+    assert np.max(np.abs(w - want)) < 1e-12 * np.max(np.abs(conn8))

@@ -1,0 +1,94 @@
+"""End-to-end parity of the whole-source pipeline with the oracle at 1e-10 (GPU).
+
+cvchip_cvc_source runs its own 120 gamma5-hermitian solves on the GPU and
+contracts them; the expected conn, contact term and Pi(q) come from the
+oracle's own 120 solves (the CPU restatement of invert_Qtm.c:276-478) fed to
+the oracle contraction (cvc_exact2_xspace.c:577-861), frozen in
+tests/golden/golden_e2e.npz (tests/golden/make_golden_e2e.py).  Both sides
+solve to a squared relative residual of 1e-28, which puts the solvers' share
+of the contraction error at the rounding level (the oracle's conn from solves
+at 1e-28 and 1e-30 differ by 4.5e-15 relative at 4^4), so the comparison
+tests the whole chain at the 1e-10 bar:
+
+  4^4   kappa 0.163, mu 0.01, gauge seed 1, source (1, 2, 3, 0)
+  8^4   the plain-method configuration: kappa 0.163, mu 0.01, gauge seed 1,
+        antiperiodic time boundary, point source at the origin; FP64 solves,
+        FP32-inner solves with FP64 reliable updates, and even/odd
+        preconditioned solves
+
+The tolerance is relative to the largest |component|: max |got - want| /
+max |want| < 1e-10 for conn, for the four contact terms and for Pi(q).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+BC = (1.0, 0.0, 0.0, 0.0)
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(os.path.join(HERE, "golden", "golden_e2e.npz"), allow_pickle=False)
+
+
+def pi_q_numpy(conn, ct, dims, src):
+    """Pi_{mu nu}(q) = sum_x e^{-iq(x-y)} e^{-i(q_mu-q_nu)/2} conn_{mu nu}(x) - delta_{mu nu} contact_nu."""
+    c = (conn[..., 0] + 1j * conn[..., 1]).reshape(tuple(dims) + (4, 4))
+    F = np.fft.fftn(c, axes=(0, 1, 2, 3))
+    ns = np.meshgrid(*[np.arange(L) for L in dims], indexing="ij")
+    q = [2 * np.pi * ns[d] / dims[d] for d in range(4)]
+    ey = np.exp(1j * sum(q[d] * src[d] for d in range(4)))
+    want = np.empty_like(F)
+    for mu in range(4):
+        for nu in range(4):
+            want[..., mu, nu] = ey * np.exp(-0.5j * (q[mu] - q[nu])) * F[..., mu, nu] - (mu == nu) * ct[nu]
+    return want.reshape(-1, 16)
+
+
+def check(ctx, conn, ct, gold, key):
+    dims = tuple(int(v) for v in gold[f"dims{key}"])
+    src = tuple(int(v) for v in gold[f"src{key}"])
+    want_c, want_ct = gold[f"conn{key}"], gold[f"contact{key}"]
+    e_conn = rel_err(conn, want_c)
+    e_ct = float(np.max(np.abs(ct - want_ct)) / np.max(np.abs(want_ct)))
+    piq = ctx.pi_q(conn, ct, src)
+    got_q = piq[..., 0] + 1j * piq[..., 1]
+    want_q = pi_q_numpy(want_c, want_ct, dims, src)
+    e_q = float(np.max(np.abs(got_q - want_q)) / np.max(np.abs(want_q)))
+    print(f"  {key}: conn {e_conn:.2e}  contact {e_ct:.2e}  Pi(q) {e_q:.2e}", flush=True)
+    assert e_conn < TOL and e_ct < TOL and e_q < TOL
+    return e_conn, e_ct, e_q
+
+
+@pytest.mark.parametrize("key", ["4", "8"])
+def test_whole_source_fp64_matches_oracle_solves(chip, gold, key):
+    dims = tuple(int(v) for v in gold[f"dims{key}"])
+    src = tuple(int(v) for v in gold[f"src{key}"])
+    print(f"e2e fp64 dims={dims} src={src}", flush=True)
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=chip.gauge_random(dims, 1, 0.3))
+    conn, ct, st = ctx.cvc_source(src, float(gold["sp"]), 20000)
+    assert st["converged"] == 120 and st["true_res2_max"] < 1e-26, st
+    check(ctx, conn, ct, gold, key)
+
+
+@pytest.mark.parametrize("prec,eo", [("f32", False), ("f32", True), ("f64", True)])
+def test_whole_source_mixed_and_eo_match_oracle_solves(chip, gold, prec, eo):
+    """The FP32-inner pipeline (FP64 reliable updates) and the even/odd
+    preconditioned pipeline against the oracle's FP64 solves at 8^4."""
+    key = "8"
+    dims = tuple(int(v) for v in gold[f"dims{key}"])
+    src = tuple(int(v) for v in gold[f"src{key}"])
+    print(f"e2e {prec} eo={eo} dims={dims}", flush=True)
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=chip.gauge_random(dims, 1, 0.3))
+    if eo:
+        ctx.set_solver_flags(chip.SOLVE_EO)
+    p = chip.PREC_F32 if prec == "f32" else chip.PREC_F64
+    conn, ct, st = ctx.cvc_source(src, float(gold["sp"]), 20000, prec=p, delta=0.01)
+    assert st["converged"] == 120 and st["true_res2_max"] < 1e-26, st
+    check(ctx, conn, ct, gold, key)

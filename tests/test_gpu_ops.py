@@ -118,11 +118,80 @@ def test_dslash_policies(chip, orc, nrhs):
     with pytest.raises(chip.CvchipError):
         ctx.set_dslash_policy(0x50)        # column z extent 8 does not divide LZ = 6
     with pytest.raises(chip.CvchipError):
-        ctx.set_dslash_policy(0x60)        # traversal mode 6 does not exist
+        ctx.set_dslash_policy(0x70)        # traversal mode 7 does not exist
+    with pytest.raises(chip.CvchipError):
+        ctx.set_dslash_policy(0x300065)    # t-march patch x extent 3 does not divide LX = 4
     with pytest.raises(chip.CvchipError):
         ctx.set_dslash_policy(0x110355)    # column y extent 3 is not a power of two
     with pytest.raises(chip.CvchipError):
         ctx.set_dslash_policy(0x350141)     # brick t extent 5 does not divide T = 16
+
+
+# t march (traversal 6): march length bits 8-15, patch x / y extents bits 16-19 / 20-23
+MARCH = [0x65, 0x0165, 0x0465, 0x110865, 0x221065, 0x420365, 0x64]
+
+
+@pytest.mark.parametrize("dims,nrhs", [((16, 4, 8, 6), 12), ((8, 8, 8, 8), 12), ((6, 4, 4, 8), 5),
+                                       ((8, 8, 8, 8), 48), ((4, 6, 4, 8), 1)])
+def test_dslash_march(chip, orc, dims, nrhs):
+    """The t-marching operator (traversal 6) gives the oracle's operator for every march
+    length (including 1 and lengths that do not divide T), patch shape, precision and
+    gamma5 output, with and without the XCD-aware patch order."""
+    print(f"dslash march dims={dims} nrhs={nrhs}", flush=True)
+    g = chip.gauge_random(dims, 17, 0.3)
+    bc = (1.0, 0.3, 0.25, -0.5)
+    ctx = chip.Context(dims, 0.1632, 0.004, bc, gauge=g)
+    L = orc.Lattice(dims, 0.1632, 0.004, bc)
+    rng = np.random.default_rng(100 + nrhs)
+    phi = rand_spinor(rng, L.V, nrhs)
+    want = [L.Q_phi(g, phi[r]) for r in range(nrhs)]
+    want5 = [L.Qf5(g, phi[r], -0.02) for r in range(nrhs)]
+    fin, fout = ctx.field(nrhs), ctx.field(nrhs)
+    fin.upload(phi)
+    for pol in MARCH:
+        if ((pol >> 16) & 15) and dims[1] % ((pol >> 16) & 15) or ((pol >> 20) & 15) and dims[2] % ((pol >> 20) & 15):
+            continue
+        ctx.set_dslash_policy(pol)
+        for recon in (18, 12):
+            ctx.set_link_recon(recon)
+            ctx.dslash(fout, fin)
+            got = fout.download()
+            for r in range(nrhs):
+                assert rel_err(got[r], want[r]) < 1e-13, (hex(pol), recon, r)
+        ctx.set_link_recon(18)
+        ctx.dslash(fout, fin, mutm=-0.02, gamma5_out=True)
+        got = fout.download()
+        for r in range(nrhs):
+            assert rel_err(got[r], want5[r]) < 1e-13, (hex(pol), "g5", r)
+        for prec, tol in [(chip.PREC_F32, 2e-6), (chip.PREC_F16, 2e-3)]:
+            a, b, back = ctx.field(nrhs, prec), ctx.field(nrhs, prec), ctx.field(nrhs)
+            a.copy_from(fin)
+            ctx.dslash(b, a)
+            back.copy_from(b)
+            lo = back.download()
+            for r in range(nrhs):
+                assert rel_err(lo[r], want[r]) < tol, (hex(pol), prec, r)
+    ctx.set_dslash_policy(0x5)
+
+
+def test_march_solves(chip, orc):
+    """Solves whose operator applications (with the fused dot products) run on the
+    t-marching kernel reach the requested residual with the oracle's operator, in FP64
+    and with FP32 / FP16 inner iterations."""
+    dims = (8, 8, 8, 8)
+    g = chip.gauge_random(dims, 1, 0.3)
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=g)
+    L = orc.Lattice(dims, 0.163, 0.01, BC)
+    ctx.set_dslash_policy(0x65)
+    b = np.stack([chip.point_source(L.V, 0, a // 3, a % 3) for a in range(12)])
+    for prec, delta in ((chip.PREC_F64, 0.1), (chip.PREC_F32, 0.01), (chip.PREC_F16, 0.1)):
+        print(f"march solve prec={prec}", flush=True)
+        rc, x, st = ctx.invert_Qtm_her(b, 1e-20, 3000, prec=prec, delta=delta)
+        assert rc > 0 and st["converged"] == 12, st
+        for a in (0, 7, 11):
+            res = L.Q_phi(g, x[a]) - b[a]
+            assert L.dot_re(res, res) < 2e-20
+    ctx.set_dslash_policy(0x5)
 
 
 def test_scalar_products(chip, orc):
