@@ -52,7 +52,7 @@ def parse():
                     help="skip the six 12-rhs solves (for profiling the whole-source run alone)")
     ap.add_argument("--no-48", action="store_true",
                     help="skip the 48-rhs batched solve (12 spin-colour x 4 source positions)")
-    ap.add_argument("--delta", type=float, default=0.01, help="reliable-update factor of the FP32 mixed solver")
+    ap.add_argument("--delta", type=float, default=0.001, help="reliable-update factor of the FP32 mixed solver")
     ap.add_argument("--delta16", type=float, default=0.1, help="reliable-update factor of the FP16 mixed solver")
     ap.add_argument("--delta16-eo", type=float, default=0.02,
                     help="reliable-update factor of the FP16 mixed solver on the even/odd system")
@@ -204,6 +204,10 @@ def main():
         for k in ("eo_f64", "eo_mixed_f32", "eo_mixed_f16"):
             if extra.get(f"solve_{k}_s"):
                 extra[f"{k}_speedup"] = round(extra["solve_f64_s"] / extra[f"solve_{k}_s"], 3)
+        # the same comparison on the even/odd preconditioned system (FP32 or FP16 inner against FP64)
+        for k in ("eo_mixed_f32", "eo_mixed_f16"):
+            if extra.get(f"solve_{k}_s") and extra.get("solve_eo_f64_s"):
+                extra[f"{k}_vs_eo_f64_speedup"] = round(extra["solve_eo_f64_s"] / extra[f"solve_{k}_s"], 3)
 
     if not a.no_solve and not a.no_48:
         # 48 rhs: 12 spin-colour x 4 source positions (seed 4) in one batched gamma5-hermitian

@@ -165,8 +165,8 @@ int download_aos(cvchip_ctx *c, const Field &f, double *host, int rhs0, int n);
 // Dslash launcher: out = D(mutm) in (gamma5_out: out = gamma5 D in).
 // Optional fused reductions, written as per-block partials into c->red:
 //   DOT_NONE, DOT_TS (<out,in> complex and <out,out>; <out,r2> when r2 is given),
-//   DOT_R2 (<r2,out> complex)
-enum DotMode { DOT_NONE = 0, DOT_TS = 1, DOT_R2 = 2 };
+//   DOT_R2 (<r2,out> complex), DOT_NN (<out,out>)
+enum DotMode { DOT_NONE = 0, DOT_TS = 1, DOT_R2 = 2, DOT_NN = 3 };
 int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_out,
            DotMode dm = DOT_NONE, const Field *r2 = nullptr, int *nblocks_out = nullptr);
 

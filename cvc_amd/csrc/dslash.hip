@@ -12,6 +12,8 @@
 // the link, and rebuilds the lower spin components from the upper ones.
 // Periodic wrap and the boundary phase are resolved in the kernel: there is no
 // halo.  Optional epilogues fuse the solver's per-rhs dot products.
+#include <algorithm>
+
 #include "spinor_io.h"
 
 namespace cvchip {
@@ -539,10 +541,16 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
         b += qx * oy - qy * ox;
       }
       dv[0] = a; dv[1] = b;
+    } else if constexpr (DM == DOT_NN) {
+      double n = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++) n += (double)acc[c].x * acc[c].x + (double)acc[c].y * acc[c].y;
+      dv[0] = n;
     }
   }
   if constexpr (DM == DOT_TS) block_reduce_store<3>(dv, part, P.N);
   else if constexpr (DM == DOT_R2) block_reduce_store<2>(dv, part, P.N);
+  else if constexpr (DM == DOT_NN) block_reduce_store<1>(dv, part, P.N);
 }
 
 // ------------------------------------------------------- t-marching operator
@@ -806,10 +814,16 @@ k_dslash_march(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, 
         b += qx * oy - qy * ox;
       }
       dv[0] += a; dv[1] += b;
+    } else if constexpr (DM == DOT_NN) {
+      double n = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++) n += (double)acc[c].x * acc[c].x + (double)acc[c].y * acc[c].y;
+      dv[0] += n;
     }
   }
   if constexpr (DM == DOT_TS) block_reduce_store<3>(dv, part, P.N);
   else if constexpr (DM == DOT_R2) block_reduce_store<2>(dv, part, P.N);
+  else if constexpr (DM == DOT_NN) block_reduce_store<1>(dv, part, P.N);
 }
 
 // ------------------------------------------------------------- launcher
@@ -858,15 +872,16 @@ struct Traversal {
 // policy give the march length M (0: 16; the largest divisor of T not above
 // it is used), bits 16-19 and 20-23 the patch's x and y extents (0: 8; a
 // value that does not divide LX or LY falls back to 1).
-static bool march_geometry(const cvchip_ctx *c, int N, Traversal &t) {
+static bool march_geometry(const cvchip_ctx *c, int N, Traversal &t, bool explicit_policy, uint32_t M_auto) {
   const Lattice &L = c->lat;
   if (N < 1 || N > 192) return false;
   uint32_t S = 192u / (uint32_t)N;
   if (S > 48) S = 48;
-  uint32_t M = (uint32_t)((c->dslash_policy >> 8) & 255);
-  if (M == 0) M = 16;
+  const int pol = explicit_policy ? c->dslash_policy : 0;
+  uint32_t M = (uint32_t)((pol >> 8) & 255);
+  if (M == 0) M = explicit_policy ? 16 : M_auto;
   while (M > 1 && L.T % (int)M) M--;
-  uint32_t XP = (uint32_t)((c->dslash_policy >> 16) & 15), YP = (uint32_t)((c->dslash_policy >> 20) & 15);
+  uint32_t XP = (uint32_t)((pol >> 16) & 15), YP = (uint32_t)((pol >> 20) & 15);
   if (XP == 0) XP = 8;
   if (YP == 0) YP = 8;
   if (L.LX % (int)XP) XP = 1;
@@ -898,7 +913,10 @@ static int log2_exact(uint32_t v) {   // -1 unless v is a power of two
 // and 20-23 the brick's t and x extents of mode 4 (0: 4); for mode 5, bits
 // 16-19 are log2 of the column's z extent (0: 8) and bits 20-23 its x extent
 // (0: 4).
-static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
+// lowp: the FP32 operator (whose automatic traversal is the t march with
+// marches of 4 slices for 8 <= N <= 24: 1.72 ms against 1.95-2.06 ms for the
+// bricks at 64x32^3 with 12 rhs; the FP64 and FP16 operators keep the bricks)
+static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false, bool lowp = false) {
   Lattice L = c->lat;
   if (eo) {   // the half lattice of one parity: LZ/2 half-sites per z line
     L.LZ /= 2;
@@ -909,8 +927,12 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
   int want = (c->dslash_policy >> 4) & 15;
   if (want == 6) {
     Traversal t{};
-    if (!eo && march_geometry(c, N, t)) return t;
+    if (!eo && march_geometry(c, N, t, true, 16)) return t;
     want = 4;   // the even/odd operator does not march
+  }
+  if (want == 0 && lowp && !eo && N >= 8 && N <= 24) {
+    Traversal t{};
+    if (march_geometry(c, N, t, false, 4)) return t;
   }
   uint32_t yb = (uint32_t)((c->dslash_policy >> 8) & 255);
   // automatic: bricks of 4 slices x 4 x-planes x yb y-rows for the multi-rhs
@@ -997,7 +1019,7 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false) {
 }
 
 unsigned dslash_blocks(const cvchip_ctx *c, int N) {
-  const unsigned a = traversal(c, N).nblk;
+  const unsigned a = std::max(traversal(c, N).nblk, traversal(c, N, false, true).nblk);
   const unsigned b = c->lat.LZ % 2 == 0 ? traversal(c, N, true).nblk : 0u;
   return a > b ? a : b;
 }
@@ -1021,7 +1043,7 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   P.hopc = (R)(eo ? eo->hopc : -0.5);
   P.dinv = (R)(1.0 / (0.25 / (c->kappa * c->kappa) + mutm * mutm));
   const int B = block_for(N);
-  const Traversal tr = traversal(c, N, EO);
+  const Traversal tr = traversal(c, N, EO, IOI::K == 2);
   P.csz = tr.csz;
   P.bpc = tr.bpc;
   P.mode = tr.mode;
@@ -1079,10 +1101,12 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
     if (g5) {
       if (dm == DOT_NONE) CVC_MLAUNCH(true, DOT_NONE);
       else if (dm == DOT_TS) CVC_MLAUNCH(true, DOT_TS);
+      else if (dm == DOT_NN) CVC_MLAUNCH(true, DOT_NN);
       else CVC_MLAUNCH(true, DOT_R2);
     } else {
       if (dm == DOT_NONE) CVC_MLAUNCH(false, DOT_NONE);
       else if (dm == DOT_TS) CVC_MLAUNCH(false, DOT_TS);
+      else if (dm == DOT_NN) CVC_MLAUNCH(false, DOT_NN);
       else CVC_MLAUNCH(false, DOT_R2);
     }
 #undef CVC_MLAUNCH
@@ -1111,10 +1135,12 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   if (g5) {
     if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);
     else if (dm == DOT_TS) CVC_LAUNCH(true, DOT_TS);
+    else if (dm == DOT_NN) CVC_LAUNCH(true, DOT_NN);
     else CVC_LAUNCH(true, DOT_R2);
   } else {
     if (dm == DOT_NONE) CVC_LAUNCH(false, DOT_NONE);
     else if (dm == DOT_TS) CVC_LAUNCH(false, DOT_TS);
+    else if (dm == DOT_NN) CVC_LAUNCH(false, DOT_NN);
     else CVC_LAUNCH(false, DOT_R2);
   }
 #undef CVC_LAUNCH

@@ -498,6 +498,10 @@ static int true_residual(Solve &S, Field &x, const Field &b, Field &tmp, double 
   return 0;
 }
 
+__global__ void k_zero_doubles(double *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.;
+}
+
 // The operator of the recurrence: D(mu), or for the gamma5-hermitian system
 // A = gamma5 D(mu) gamma5 D(-mu) = Qf5(mu) Qf5(-mu) (invert_Qtm.c:325-327).
 struct Op {
@@ -513,6 +517,25 @@ struct Op {
     CVC_TRY(dslash_eo(c, *tmp, in, nullptr, mutm, 0, 1, 0.5, 0));
     return dslash_eo(c, out, *tmp, &in, mutm, 1, 0, -0.5, g5, dm,
                      dm == DOT_TS ? (partner ? partner : &in) : partner, nb);
+  }
+  // t = A in with <t,in> (complex) in res[0..N) / res[N..2N) and <t,t> in
+  // res[2N..3N).  For the gamma5-hermitian operator A = B^dagger B with
+  // B = gamma5 D(-mu) (gamma5 M(-mu) on the even/odd system), so
+  // <t,in> = <B in, B in> is the norm of the first application's output: both
+  // applications fuse a norm, and neither reads the input again.
+  int apply_ts(Field &out, const Field &in, cvchip_ctx *cc, int N, double *res) {
+    int nb1 = 0, nb2 = 0;
+    if (!her) {
+      CVC_TRY(apply(out, in, DOT_TS, nullptr, &nb1));
+      return reduce_partials(cc, (const double *)cc->red.ptr, nb1, 3, N, res);
+    }
+    if (eo) CVC_TRY(applyM(*aux, in, -c->mu, 1, DOT_NN, nullptr, &nb1));
+    else CVC_TRY(dslash(c, *aux, in, -c->mu, 1, DOT_NN, nullptr, &nb1));
+    CVC_TRY(reduce_partials(cc, (const double *)cc->red.ptr, nb1, 1, N, res));
+    hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(256), 0, cc->stream, res + N, N);
+    if (eo) CVC_TRY(applyM(out, *aux, c->mu, 1, DOT_NN, nullptr, &nb2));
+    else CVC_TRY(dslash(c, out, *aux, c->mu, 1, DOT_NN, nullptr, &nb2));
+    return reduce_partials(cc, (const double *)cc->red.ptr, nb2, 1, N, res + 2 * N);
   }
   int apply(Field &out, const Field &in, DotMode dm, const Field *r2, int *nb) {
     if (eo) {
@@ -594,9 +617,8 @@ static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
     for (int it = 0; it <= niter_max; it++) {
       hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
       hipLaunchKernelGGL((k_s_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, is, ir1, ip2, S.st, S.VN, N, S.nt);
-      TRY(op.apply(t, s, DOT_TS, nullptr, &nb));
+      TRY(op.apply_ts(t, s, c, N, S.res));
       dcalls++;
-      TRY(reduce(S, nb, 3));
       hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
       hipLaunchKernelGGL((k_r1_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, is, iT, ir2, S.st,
                          S.VN, N, (double *)c->red.ptr, S.nt);
@@ -721,9 +743,8 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
     for (int it = 0; it <= niter_max; it++) {
       hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
       hipLaunchKernelGGL((k_s_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, is, ir1, ip2, S.st, S.VN, N, S.nt);
-      TRY(opl.apply(t, s, DOT_TS, nullptr, &nb));
+      TRY(opl.apply_ts(t, s, c, N, S.res));
       dcalls++;
-      TRY(reduce(S, nb, 3));
       hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
       hipLaunchKernelGGL((k_r1_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, is, iT, ir2, S.st,
                          S.VN, N, (double *)c->red.ptr, S.nt);
