@@ -20,9 +20,12 @@
 //
 // GPU schedule, per nu:
 //   k_psi:   Psi1_k(x) = sum_l conj(G1_kl) Sd_nu_l(x),  Psi2_k(x) = sum_l conj(G2_kl) Sd_4_l(x)
-//            (one thread per (x, k); stored as a 24-column field)
-//   k_conn:  one thread per (x, mu, k): the four terms for column k, summed over
-//            k in LDS, written as conn_{mu nu}(x) * (-1/4).
+//            (one thread per (x, part, k); stored as a 24-column field)
+//   k_conn:  one thread per (x, part, k) with its own Psi and up column in
+//            registers for the four mu: the terms as <(gamma_mu -+ 1) gamma5 Psi,
+//            U phi> (the sink vertex applied to Psi, the link to phi two spin
+//            rows at a time), summed over (part, k) in LDS in a fixed order,
+//            written as conn_{mu nu}(x) * (-1/4).
 // FP64 throughout.
 #include <stdio.h>
 #include <string.h>
@@ -37,99 +40,171 @@ int invert_her_fields(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
                       double delta, unsigned flags, cvchip_stats *stats);
 
 // Psi fields for one nu.  up/dn: 60-column sets (12*s + k); psi: 24-column set.
-__global__ void __launch_bounds__(256) k_psi(double2 *__restrict__ psi, const double2 *__restrict__ dn,
+// A block of 240 threads handles 10 sites: the 24 down-propagator columns the
+// block needs (12 from y + nu, 12 from y, all 12 components) are staged in LDS
+// with coalesced loads, each read once; then one thread per (x, part, k)
+// (k fastest): part 0 is Psi1 from the columns of y + nu, part 1 Psi2 from y.
+constexpr int kPsiSites = 10;
+__global__ void __launch_bounds__(240) k_psi(double2 *__restrict__ psi, const double2 *__restrict__ dn,
                                              SrcLinks L, int nu, uint32_t V) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;   // (x, k), k fastest
-  if (i >= V * 12u) return;
-  const uint32_t x = i / 12u;
-  const int k = (int)(i - x * 12u);
-  const int ir = k / 3, ia = k % 3;
+  __shared__ double2 sd[kPsiSites][12][24];
+  const uint32_t x0 = blockIdx.x * (uint32_t)kPsiSites;
   const uint32_t VN = V * 60u, VP = V * 24u;
+  for (uint32_t e = threadIdx.x; e < (uint32_t)kPsiSites * 12u * 24u; e += blockDim.x) {
+    const uint32_t col = e % 24u, rem = e / 24u, c = rem % 12u, xs = rem / 12u;
+    const uint32_t x = x0 + xs;
+    if (x < V) sd[xs][c][col] = dn[(size_t)c * VN + (size_t)x * 60u + (col < 12u ? 12u * (uint32_t)nu + col : 36u + col)];
+  }
+  __syncthreads();
+  const uint32_t xs = threadIdx.x / 24u, x = x0 + xs;
+  if (x >= V) return;
+  const int j24 = (int)(threadIdx.x % 24u), part = j24 / 12, k = j24 % 12;
+  const int ir = k / 3, ia = k % 3;
   // (gamma_nu gamma5)_{ir, j}: j = GCOL[nu][ir], value = GVAL[nu][ir] * g5sign(j)
   const int j = GCOL[nu][ir];
   double2 e = unit_val(GVAL[nu][ir]);
   if (g5sign(j) < 0) { e.x = -e.x; e.y = -e.y; }
   const double s5 = (double)g5sign(ir);
   const double2 *Us = L.u[nu];
-  double2 p1[12], p2[12];
+  double2 p[12];
 #pragma unroll
-  for (int c = 0; c < 12; c++) p1[c] = p2[c] = make_double2(0, 0);
+  for (int c = 0; c < 12; c++) p[c] = make_double2(0, 0);
   for (int ib = 0; ib < 3; ib++) {
-    // part 1: conj(G1_{k,l}) with G1 = (e d_{l,j} - s5 d_{l,ir}) Us[ia][ib]
-    const double2 u1 = Us[3 * ia + ib];
-    const double2 w1a = cmulc(u1, cmulc(e, make_double2(1, 0)));      // conj(e * u1)
-    const double2 w1b = make_double2(-s5 * u1.x, s5 * u1.y);          // conj(-s5 * u1)
-    // part 2: G2 = (e d_{l,j} + s5 d_{l,ir}) conj(Us[ib][ia]); conj(G2) = conj(e) Us[ib][ia], s5 Us[ib][ia]
-    const double2 u2 = Us[3 * ib + ia];
-    const double2 w2a = cmulc(e, u2);
-    const double2 w2b = make_double2(s5 * u2.x, s5 * u2.y);
-    const int la = 3 * j + ib, lb = 3 * ir + ib;
-#pragma unroll
-    for (int c = 0; c < 12; c++) {
-      const size_t pl = (size_t)c * VN + (size_t)x * 60u;
-      const double2 a1 = dn[pl + 12 * nu + la], b1 = dn[pl + 12 * nu + lb];
-      const double2 a2 = dn[pl + 48 + la], b2 = dn[pl + 48 + lb];
-      p1[c] = cfma(cfma(p1[c], w1a, a1), w1b, b1);
-      p2[c] = cfma(cfma(p2[c], w2a, a2), w2b, b2);
+    double2 wa, wb;
+    if (part == 0) {
+      // conj(G1_{k,l}) with G1 = (e d_{l,j} - s5 d_{l,ir}) Us[ia][ib]
+      const double2 u1 = Us[3 * ia + ib];
+      wa = cmulc(u1, cmulc(e, make_double2(1, 0)));      // conj(e * u1)
+      wb = make_double2(-s5 * u1.x, s5 * u1.y);          // conj(-s5 * u1)
+    } else {
+      // G2 = (e d_{l,j} + s5 d_{l,ir}) conj(Us[ib][ia]); conj(G2) = conj(e) Us[ib][ia], s5 Us[ib][ia]
+      const double2 u2 = Us[3 * ib + ia];
+      wa = cmulc(e, u2);
+      wb = make_double2(s5 * u2.x, s5 * u2.y);
     }
+    const int la = 12 * part + 3 * j + ib, lb = 12 * part + 3 * ir + ib;
+#pragma unroll
+    for (int c = 0; c < 12; c++) p[c] = cfma(cfma(p[c], wa, sd[xs][c][la]), wb, sd[xs][c][lb]);
+#pragma unroll
+    for (int c = 0; c < 12; c++) pin(p[c]);
   }
 #pragma unroll
-  for (int c = 0; c < 12; c++) {
-    psi[(size_t)c * VP + (size_t)x * 24u + k] = p1[c];
-    psi[(size_t)c * VP + (size_t)x * 24u + 12 + k] = p2[c];
+  for (int c = 0; c < 12; c++) psi[(size_t)c * VP + (size_t)x * 24u + j24] = p[c];
+}
+
+// acc += <Psi, G U phi> for one term, G = gamma5 (gamma_MU + SGN) the sink
+// vertex, computed as <(gamma_MU + SGN) gamma5 Psi, U phi> (the vertex moved
+// onto Psi; DAG: U^dagger).  gamma_MU couples the spin rows in two pairs
+// (a, GCOL[MU][a]), done one pair at a time.  OWN_PSI: Psi is the thread's own
+// (registers) and phi is read at nb (plane stride VS), else phi is the own and
+// Psi is read at nb.
+template <int MU, int SGN, bool DAG, bool OWN_PSI>
+__device__ __forceinline__ void conn_term(double2 &acc, const double2 own[12], const double2 *__restrict__ nb,
+                                          uint32_t VS, const double2 u[9]) {
+#pragma unroll
+  for (int pr = 0; pr < 2; pr++) {
+    const int a0 = pr, a1 = GCOL[MU][pr];
+    double2 p0[3], p1[3], f0[3], f1[3];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+      if (OWN_PSI) {
+        p0[cc] = own[3 * a0 + cc];
+        p1[cc] = own[3 * a1 + cc];
+        f0[cc] = nb[(size_t)(3 * a0 + cc) * VS];
+        f1[cc] = nb[(size_t)(3 * a1 + cc) * VS];
+      } else {
+        p0[cc] = nb[(size_t)(3 * a0 + cc) * VS];
+        p1[cc] = nb[(size_t)(3 * a1 + cc) * VS];
+        f0[cc] = own[3 * a0 + cc];
+        f1[cc] = own[3 * a1 + cc];
+      }
+    }
+    // v_a = e_a (gamma5 Psi)_{GCOL a} + SGN (gamma5 Psi)_a  for a = a0, a1
+    const double2 e0 = unit_val(GVAL[MU][a0]), e1 = unit_val(GVAL[MU][a1]);
+    const double s0 = (double)g5sign(a0), s1 = (double)g5sign(a1);
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+      const double2 g0 = make_double2(s0 * p0[ii].x, s0 * p0[ii].y), g1 = make_double2(s1 * p1[ii].x, s1 * p1[ii].y);
+      const double2 q0 = cmul(e0, g1), q1 = cmul(e1, g0);
+      const double2 v0 = make_double2(q0.x + SGN * g0.x, q0.y + SGN * g0.y);
+      const double2 v1 = make_double2(q1.x + SGN * g1.x, q1.y + SGN * g1.y);
+      double2 w0 = make_double2(0, 0), w1 = make_double2(0, 0);
+#pragma unroll
+      for (int jj = 0; jj < 3; jj++) {
+        if (DAG) {
+          w0 = cfmac(w0, u[3 * jj + ii], f0[jj]);
+          w1 = cfmac(w1, u[3 * jj + ii], f1[jj]);
+        } else {
+          w0 = cfma(w0, u[3 * ii + jj], f0[jj]);
+          w1 = cfma(w1, u[3 * ii + jj], f1[jj]);
+        }
+      }
+      acc = cfmac(acc, v0, w0);
+      acc = cfmac(acc, v1, w1);
+    }
+    pin(acc);
   }
 }
 
-// one thread per (x, mu, k); blocks of 240 = 5 sites x 4 mu x 12 k
+// The two terms of one mu for the thread's own Psi (ops) and up column (oph) at
+// site x: <Psi(x), Gm U phi(x+mu)> + <Psi(x+mu), Gp U^dagger phi(x)>.
+template <int MU>
+__device__ __forceinline__ void conn_mu(double2 &acc, const double2 ops[12], const double2 oph[12],
+                                        const double2 *__restrict__ up, const double2 *__restrict__ psi,
+                                        const double2 *__restrict__ U, const ConnParams &P, uint32_t x,
+                                        uint32_t coord, uint32_t col_up, uint32_t col_psi) {
+  const uint32_t L = (uint32_t)(MU == 0 ? P.T : MU == 1 ? P.LX : MU == 2 ? P.LY : P.LZ);
+  const uint32_t stride = MU == 0 ? P.sT : MU == 1 ? P.sX : MU == 2 ? (uint32_t)P.LZ : 1u;
+  const uint32_t xp = (coord == L - 1) ? x - (L - 1) * stride : x + stride;
+  double2 u[9];
+  const double2 ph = make_double2(P.ph_re[MU], P.ph_im[MU]);
+#pragma unroll
+  for (int q = 0; q < 9; q++) u[q] = cmul(U[(size_t)(MU * 9 + q) * P.V + x], ph);
+  conn_term<MU, -1, false, true>(acc, ops, up + (size_t)xp * 60u + col_up, P.V * 60u, u);
+  conn_term<MU, +1, true, false>(acc, oph, psi + (size_t)xp * 24u + col_psi, P.V * 24u, u);
+}
+
+// One thread per (x, part, k) (k fastest), blocks of 240 = 10 sites x 24: the
+// thread keeps its own Psi_part(x)_k and up column phi_s(x)_k (s = y for part 1,
+// y + nu for part 2) in registers for the four mu and reads only the neighbour
+// columns at x + mu.  The 24 values of each (x, mu) are summed in LDS in a
+// fixed order and written as conn_{mu nu}(x) * (-1/4).
 __global__ void __launch_bounds__(240) k_conn(double2 *__restrict__ conn, const double2 *__restrict__ up,
                                               const double2 *__restrict__ psi, const double2 *__restrict__ U,
                                               ConnParams P, int nu) {
-  __shared__ double2 red[240];
+  __shared__ double2 red[4][240];
   const int t = threadIdx.x;
-  const uint32_t x = blockIdx.x * 5u + (uint32_t)(t / 48);
-  const int mu = (t % 48) / 12;
-  const int k = t % 12;
-  double2 acc = make_double2(0, 0);
+  const int j24 = t % 24, part = j24 / 12, k = j24 % 12;
+  const uint32_t x = blockIdx.x * 10u + (uint32_t)(t / 24);
+  double2 acc[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
   if (x < P.V) {
     uint32_t rem = x;
     const uint32_t ct = rem / P.sT; rem -= ct * P.sT;
     const uint32_t cx = rem / P.sX; rem -= cx * P.sX;
     const uint32_t cy = rem / (uint32_t)P.LZ;
     const uint32_t cz = rem - cy * (uint32_t)P.LZ;
-    uint32_t coord, L, stride;
-    if (mu == 0) { coord = ct; L = P.T; stride = P.sT; }
-    else if (mu == 1) { coord = cx; L = P.LX; stride = P.sX; }
-    else if (mu == 2) { coord = cy; L = P.LY; stride = P.LZ; }
-    else { coord = cz; L = P.LZ; stride = 1; }
-    const uint32_t xp = (coord == L - 1) ? x - (L - 1) * stride : x + stride;
-    double2 u[9];
-    const double2 ph = make_double2(P.ph_re[mu], P.ph_im[mu]);
-#pragma unroll
-    for (int q = 0; q < 9; q++) u[q] = cmul(U[(size_t)(mu * 9 + q) * P.V + x], ph);
     const uint32_t VN = P.V * 60u, VP = P.V * 24u;
-    double2 phi[12], w[12], v[12], ps[12];
-    // four terms: (part, fwd/bwd)
-    for (int term = 0; term < 4; term++) {
-      const int part = term >> 1, bwd = term & 1;
-      const int s = part == 0 ? 4 : nu;                  // up propagator source
-      const uint32_t xs = bwd ? x : xp;                  // site of the up propagator
-      const uint32_t xq = bwd ? xp : x;                  // site of Psi
+    // part 1: up from y (s = 4), Psi1; part 2: up from y + nu, Psi2
+    const uint32_t col_up = (part == 0 ? 48u : 12u * (uint32_t)nu) + (uint32_t)k;
+    const uint32_t col_psi = (uint32_t)j24;
+    double2 ops[12], oph[12];
 #pragma unroll
-      for (int c = 0; c < 12; c++) phi[c] = up[(size_t)c * VN + (size_t)xs * 60u + 12 * s + k];
+    for (int c = 0; c < 12; c++) ops[c] = psi[(size_t)c * VP + (size_t)x * 24u + col_psi];
 #pragma unroll
-      for (int c = 0; c < 12; c++) ps[c] = psi[(size_t)c * VP + (size_t)xq * 24u + 12 * part + k];
-      su3_mul(w, u, phi, bwd != 0);
-      sink_vertex(v, w, mu, bwd ? 1.0 : -1.0);
-      const double2 d = dot12(ps, v);
-      acc.x += d.x;
-      acc.y += d.y;
-    }
+    for (int c = 0; c < 12; c++) oph[c] = up[(size_t)c * VN + (size_t)x * 60u + col_up];
+    // z first: the z neighbour's columns were read moments ago by the next site's threads
+    conn_mu<3>(acc[3], ops, oph, up, psi, U, P, x, cz, col_up, col_psi);
+    conn_mu<2>(acc[2], ops, oph, up, psi, U, P, x, cy, col_up, col_psi);
+    conn_mu<1>(acc[1], ops, oph, up, psi, U, P, x, cx, col_up, col_psi);
+    conn_mu<0>(acc[0], ops, oph, up, psi, U, P, x, ct, col_up, col_psi);
   }
-  red[t] = acc;
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) red[mu][t] = acc[mu];
   __syncthreads();
-  if (k == 0 && x < P.V) {
+  if (j24 < 4 && x < P.V) {
+    const int mu = j24, t0 = t - j24;
     double2 s = make_double2(0, 0);
-    for (int q = 0; q < 12; q++) { s.x += red[t + q].x; s.y += red[t + q].y; }
+    for (int q = 0; q < 24; q++) { s.x += red[mu][t0 + q].x; s.y += red[mu][t0 + q].y; }
     conn[(size_t)x * 16 + 4 * mu + nu] = make_double2(-0.25 * s.x, -0.25 * s.y);
   }
 }
@@ -187,8 +262,8 @@ int source_links(cvchip_ctx *c, const int src[4], SrcLinks *out) {
 // Psi1 / Psi2 of one nu (24-column set psi) from the down propagators
 int launch_psi(cvchip_ctx *c, double2 *psi, const Field &dn, const SrcLinks &L, int nu) {
   const uint32_t V = (uint32_t)c->lat.V;
-  hipLaunchKernelGGL(k_psi, dim3((V * 12u + 255) / 256), dim3(256), 0, c->stream, psi, (const double2 *)dn.data, L,
-                     nu, V);
+  hipLaunchKernelGGL(k_psi, dim3((V + kPsiSites - 1) / kPsiSites), dim3(240), 0, c->stream, psi,
+                     (const double2 *)dn.data, L, nu, V);
   CVC_HIP(hipGetLastError());
   return 0;
 }
@@ -229,9 +304,9 @@ int contract_fields(cvchip_ctx *c, double *conn, double contact[8], const Field 
     if (e != hipSuccess) rc = fail(e, "source links");
   }
   for (int nu = 0; nu < 4 && !rc; nu++) {
-    hipLaunchKernelGGL(k_psi, dim3((V * 12u + 255) / 256), dim3(256), 0, c->stream, psi, (const double2 *)dn.data,
-                       hL, nu, V);
-    hipLaunchKernelGGL(k_conn, dim3((V + 4) / 5), dim3(240), 0, c->stream, dconn, (const double2 *)up.data,
+    hipLaunchKernelGGL(k_psi, dim3((V + kPsiSites - 1) / kPsiSites), dim3(240), 0, c->stream, psi,
+                       (const double2 *)dn.data, hL, nu, V);
+    hipLaunchKernelGGL(k_conn, dim3((V + 9u) / 10u), dim3(240), 0, c->stream, dconn, (const double2 *)up.data,
                        (const double2 *)psi, (const double2 *)c->gauge64, P, nu);
     e = hipGetLastError();
     if (e != hipSuccess) rc = fail(e, "contraction kernels");

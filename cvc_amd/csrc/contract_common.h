@@ -54,6 +54,12 @@ __device__ __forceinline__ double2 dot12(const double2 a[12], const double2 b[12
   return s;
 }
 
+// Bounds the compiler's load hoisting: the value is taken as read and written
+// here and memory as clobbered, so loads that follow cannot move above it (the
+// contraction kernels otherwise issue every load of a thread first and run at
+// one wave per SIMD).
+__device__ __forceinline__ void pin(double2 &v) { asm volatile("" : "+v"(v.x), "+v"(v.y) : : "memory"); }
+
 struct ConnParams {
   uint32_t V, sX, sT;
   int T, LX, LY, LZ;

@@ -128,6 +128,10 @@ struct cvchip_ctx {
   // multi-rhs operator staging: bit 0 links in LDS, bit 1 own + z-neighbour spinors in LDS
   int dslash_policy = 5;   // links staged in LDS, streaming stores, automatic traversal
   unsigned solve_flags = 0;   // OR-ed into every solve
+  // while a solver runs: per-rhs flags (op_mask[r * op_mask_stride] != 0:
+  // right-hand side r has converged); the operator skips those rhs
+  const int *op_mask = nullptr;
+  int op_mask_stride = 0;
   // staging for host <-> device transfers (AoS reference layout)
   double *stage = nullptr;
   size_t stage_bytes = 0;

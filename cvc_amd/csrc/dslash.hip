@@ -63,6 +63,10 @@ struct DParams {
   R hopc, dinv;   // dinv = 1 / ((1/(2 kappa))^2 + mutm^2)
   R k2, mutm;
   cplx<R> ph[4];
+  // converged right-hand sides of a running solve (done[r * dstride] != 0):
+  // their lanes compute and store nothing
+  const int *done;
+  int dstride;
 };
 
 // Spin projection of a neighbour spinor: h[a] = psi_a -+ g_a psi_{GCOL[a]}
@@ -380,7 +384,7 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
   const uint32_t r = threadIdx.x % (uint32_t)P.N;
   const uint32_t sl = threadIdx.x / (uint32_t)P.N;
   uint32_t vs;
-  const bool active = slot_v(sl, vs);
+  const bool active = slot_v(sl, vs) && !(P.done && P.done[(size_t)r * P.dstride]);
   const uint32_t i = vs * (uint32_t)P.N + r;
   double dv[3] = {0., 0., 0.};
   // FP32 / FP16 with 2-row links: the staged links are whole (third row rebuilt)
@@ -677,7 +681,7 @@ k_dslash_march(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, 
     return true;
   };
   uint32_t sxyz = 0;
-  const bool active = column(cl, sxyz);
+  const bool active = column(cl, sxyz) && !(P.done && P.done[(size_t)r * P.dstride]);
   uint32_t cx = 0, cy = 0, cz = 0;
   if (active) {
     cz = sxyz % (uint32_t)P.LZ;
@@ -1073,6 +1077,8 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   P.mNR = tr.mNR;
   P.mNPX = tr.mNPX;
   P.mXcd = tr.mXcd;
+  P.done = c->op_mask;
+  P.dstride = c->op_mask_stride;
   // streaming stores only for outputs that cannot stay in the 256 MB MALL
   // (bit 3: for every output)
   P.nt = (((c->dslash_policy >> 2) & 1) && out.bytes >= ((size_t)256 << 20)) || ((c->dslash_policy >> 3) & 1);

@@ -433,6 +433,22 @@ struct Solve {
   int nt = 0;                 // streaming stores in the vector kernels
 };
 
+// The operator applications of a running solve skip converged right-hand
+// sides (their results are not read again); cleared before the applications
+// that need every rhs (true residual, solution assembly) and on exit.
+struct OpMask {
+  cvchip_ctx *c;
+  OpMask(cvchip_ctx *cc, const RhsState *st) : c(cc) {
+    c->op_mask = &st->done;
+    c->op_mask_stride = (int)(sizeof(RhsState) / sizeof(int));
+  }
+  void clear() {
+    c->op_mask = nullptr;
+    c->op_mask_stride = 0;
+  }
+  ~OpMask() { clear(); }
+};
+
 // V: sites of the solver's vectors (the lattice, or one parity of it)
 static int solve_setup(cvchip_ctx *c, Solve &S, int N, int64_t V = 0) {
   S.c = c;
@@ -607,6 +623,7 @@ static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
                      S.st, 0, S.VN, N, (double *)c->red.ptr);
   TRY(reduce(S, (int)S.nblk, 2));
   hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, sp, 0);
+  OpMask mask(c, S.st);
   TRY(pull_state(S));
   all_done = 1;
   for (int r = 0; r < N; r++) all_done &= S.hst[r].done;
@@ -650,6 +667,7 @@ static int bicgstab64(cvchip_ctx *c, Field &x, const Field &b, double sp, int ni
     }
   }
   double tr = 0.;
+  mask.clear();
   if (!her && !eo) {
     TRY(true_residual(S, x, b, t, c->mu, &tr, nullptr));
     dcalls++;
@@ -733,6 +751,7 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
   TRY(reduce(S, (int)S.nblk, 2));
   hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, sp, 1);
   CVC_HIP(hipMemsetAsync(xl.data, 0, xl.bytes, c->stream));
+  OpMask mask(c, S.st);
   TRY(pull_state(S));
   all_done = 1;
   for (int r = 0; r < N; r++) all_done &= S.hst[r].done;
@@ -827,6 +846,7 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
     CVC_HIP(hipGetLastError());
   }
   double tr = 0.;
+  mask.clear();
   if (!her && !eo) {
     TRY(true_residual(S, x, b, t64, c->mu, &tr, nullptr));
     dcalls++;
