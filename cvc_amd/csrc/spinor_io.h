@@ -138,15 +138,53 @@ struct IOHalf {
 };
 
 // Sums Q per-thread values over the threads of a block that share a
-// right-hand side (thread t holds rhs t % N; blockDim.x is a multiple of N)
-// with a fixed-order tree, and writes part[(q*N + rhs)*gridDim.x + block], so
-// that the second stage reads each (q, rhs) row contiguously.
+// right-hand side (thread t holds rhs t % N; blockDim.x is a multiple of N and
+// at most 256) in a fixed order, and writes part[(q*N + rhs)*gridDim.x + block],
+// so that the second stage reads each (q, rhs) row contiguously.
+// N <= 64: lanes l and l + N of a wave hold the same rhs, so a cross-lane tree
+// with offsets N, 2N, 4N, ... (ds_bpermute shuffles, no LDS traffic and no
+// barrier) leaves lane l < N of each wave with the wave's sum for its rhs; the
+// per-wave sums meet in LDS once and one thread per rhs adds them in wave order.
+// N > 64: a tree over the block's rows in LDS.
 template <int Q>
 __device__ __forceinline__ void block_reduce_store(const double v[Q], double *__restrict__ part, int N) {
   const uint32_t slot = blockIdx.x, nb = gridDim.x;
-  __shared__ double sh[Q][256];
   const int tid = threadIdx.x;
   const int B = blockDim.x;
+  if (N <= 64) {
+    __shared__ double wsum[Q][4][64];   // [q][wave][rhs]
+    const int lane = tid & 63, w = tid >> 6, nw = (B + 63) >> 6;
+    const int nl = min(64, B - 64 * w);    // lanes of this wave
+    double s[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) s[q] = v[q];
+    for (int off = N; off < 64; off <<= 1) {
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const double o = __shfl_down(s[q], (unsigned)off, 64);
+        if (lane + off < nl) s[q] += o;
+      }
+    }
+    // zero the table (a partial last wave may hold fewer than N rhs), then fill
+    for (int e = tid; e < Q * nw * 64; e += B) (&wsum[0][0][0])[(e / (nw * 64)) * 256 + e % (nw * 64)] = 0.;
+    __syncthreads();
+    if (lane < N && lane < nl) {
+      const int r = (64 * w + lane) % N;
+#pragma unroll
+      for (int q = 0; q < Q; q++) wsum[q][w][r] = s[q];
+    }
+    __syncthreads();
+    if (tid < N) {
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        double a = 0.;
+        for (int ww = 0; ww < nw; ww++) a += wsum[q][ww][tid];
+        part[((size_t)q * N + tid) * nb + slot] = a;
+      }
+    }
+    return;
+  }
+  __shared__ double sh[Q][256];
 #pragma unroll
   for (int q = 0; q < Q; q++) sh[q][tid] = v[q];
   __syncthreads();
