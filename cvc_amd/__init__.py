@@ -28,7 +28,7 @@ EXPORTS = [
     "cvchip_lime_write_gauge", "cvchip_lime_read_spinor", "cvchip_lime_write_spinor",
     "cvchip_lime_write_propagator", "cvchip_cvc_propagators", "cvchip_field_point_sources", "cvchip_field_zero",
     "cvchip_field_copy_columns", "cvchip_gamma_tables", "cvchip_contract_twopoint",
-    "cvchip_field_contract_twopoint",
+    "cvchip_field_contract_twopoint", "cvchip_field_contract_twopoint_noalloc",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
@@ -124,6 +124,8 @@ def lib():
                                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cvchip_field_contract_twopoint.argtypes = [_vp, _dp, C.c_int, C.c_int, _vp, _vp, C.c_int,
                                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cvchip_field_contract_twopoint_noalloc.argtypes = [_vp, C.c_void_p, C.c_int, C.c_int, _vp, _vp, C.c_int,
+                                                             C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -437,6 +439,14 @@ class Context:
                                                     int(n_c), (C.c_int * 3)(*snk_mom) if snk_mom is not None else None,
                                                     (C.c_int * 4)(*src)))
         return out[:, 0] + 1j * out[:, 1]
+
+    def field_contract_twopoint_noalloc(self, contr_dev: int, idsource, idsink, chi: Field, phi: Field, n_c=3,
+                                        snk_mom=None, src=(0, 0, 0, 0)):
+        """Adds C(t) to the device buffer at address contr_dev (2 T doubles) without any
+        allocation (see include/cvchip.h)."""
+        _check(lib().cvchip_field_contract_twopoint_noalloc(
+            self._h, C.c_void_p(int(contr_dev)), int(idsource), int(idsink), chi._h, phi._h, int(n_c),
+            (C.c_int * 3)(*snk_mom) if snk_mom is not None else None, (C.c_int * 4)(*src)))
 
     def cvc_contract(self, props, src):
         assert len(props) == 120

@@ -373,6 +373,13 @@ extern "C" int cvchip_ctx_create(cvchip_ctx **ctx, const int dims[4], const doub
     c->ph.im[m] = sin(bc_angle[m] * M_PI / (double)dims[m]);
   }
   hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) {
+    const int rs = twopoint_scratch_init(c->device);
+    if (rs) {
+      cvchip_ctx_destroy(c);
+      return rs;
+    }
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);

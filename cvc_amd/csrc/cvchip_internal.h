@@ -148,6 +148,9 @@ struct cvchip_ctx {
 namespace cvchip {
 
 int ensure_stage(cvchip_ctx *c, size_t bytes);
+// device address of the allocation-free two-point entry point's scratch on
+// `device` (the current device), taken once per device
+int twopoint_scratch_init(int device);
 int ensure_red(cvchip_ctx *c, size_t bytes);
 int ensure_red2(cvchip_ctx *c, size_t bytes);
 // sites: 0 for the whole lattice, or the half lattice V/2 of the even/odd solver

@@ -10,6 +10,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "spinor_io.h"
@@ -122,19 +123,58 @@ __global__ void __launch_bounds__(256) k_twopoint(const double2 *__restrict__ ch
   }
 }
 
+// Per time slice t: contr[2 t], contr[2 t + 1] += the sums of the block partials
+// of slice t, in chunk order (the host loop of twopoint_fields, on the device).
+__global__ void k_twopoint_final(const double *__restrict__ part, double *__restrict__ contr, int T, int nchunk) {
+  for (int t = threadIdx.x; t < T; t += blockDim.x) {
+    double re = 0., im = 0.;
+    for (int k = 0; k < nchunk; k++) {
+      re += part[2 * ((size_t)t * nchunk + k)];
+      im += part[2 * ((size_t)t * nchunk + k) + 1];
+    }
+    contr[2 * t] += re;
+    contr[2 * t + 1] += im;
+  }
+}
+
+// Scratch of the allocation-free entry point cvchip_field_contract_twopoint_noalloc:
+// the per-block partial sums of k_twopoint (two doubles per block), at most
+// kTwoPtScratch doubles (2 MiB, 131072 blocks: T * ceil(LX LY LZ / 256) of them).
+// Each GPU holds its own copy until the process ends; its device address is
+// taken once per device when a context is created there.
+constexpr size_t kTwoPtScratch = 262144;
+static __device__ __align__(256) double scratch[kTwoPtScratch];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+int twopoint_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d beyond the %d devices the library supports", device, kMaxDevices);
+    return CVCHIP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return 0;
+  double *ptr = nullptr;
+  CVC_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
 namespace {
 
-int twopoint_fields(cvchip_ctx *c, double *contr, int idsource, int idsink, const Field &chi, const Field &phi,
-                    int n_c, const int *snk_mom, const int src[4]) {
+int twopoint_params(cvchip_ctx *c, int idsource, int idsink, const Field &chi, const Field &phi, int n_c,
+                    const int *snk_mom, const int src[4], TwoPt &P, size_t &nblk) {
   if (idsource < 0 || idsource > 15 || idsink < 0 || idsink > 15 || n_c < 1 || n_c > 3 ||
-      chi.nrhs != 4 * n_c || phi.nrhs != 4 * n_c || chi.prec != CVCHIP_PREC_F64 || phi.prec != CVCHIP_PREC_F64) {
-    set_error("two-point function: gammas 0..15, n_c 1..3, FP64 sets of 4 n_c propagator columns");
+      chi.nrhs != 4 * n_c || phi.nrhs != 4 * n_c || chi.prec != CVCHIP_PREC_F64 || phi.prec != CVCHIP_PREC_F64 ||
+      chi.V != c->lat.V || phi.V != c->lat.V) {
+    set_error("two-point function: gammas 0..15, n_c 1..3, FP64 sets of 4 n_c propagator columns on the "
+              "context's lattice");
     return CVCHIP_ERR_ARG;
   }
   const Lattice &L = c->lat;
   UMat g[16];
   gamma_mats(g);
-  TwoPt P;
   // gamma5 Gamma_snk
   const UMat M = umul(g[5], g[idsink]);
   for (int a = 0; a < 4; a++)
@@ -165,7 +205,16 @@ int twopoint_fields(cvchip_ctx *c, double *contr, int idsource, int idsink, cons
   P.VN = (uint32_t)(L.V * P.N);
   P.vol3 = (uint32_t)(L.LX * L.LY * L.LZ);
   P.nchunk = (int)((P.vol3 + 255) / 256);
-  const size_t nblk = (size_t)L.T * P.nchunk;
+  nblk = (size_t)L.T * P.nchunk;
+  return 0;
+}
+
+int twopoint_fields(cvchip_ctx *c, double *contr, int idsource, int idsink, const Field &chi, const Field &phi,
+                    int n_c, const int *snk_mom, const int src[4]) {
+  TwoPt P;
+  size_t nblk = 0;
+  CVC_TRY(twopoint_params(c, idsource, idsink, chi, phi, n_c, snk_mom, src, P, nblk));
+  const Lattice &L = c->lat;
   CVC_TRY(ensure_red2(c, sizeof(double) * 2 * nblk));
   double *part = (double *)c->red2.ptr;
   hipLaunchKernelGGL(k_twopoint, dim3((unsigned)nblk), dim3(256), 0, c->stream, (const double2 *)chi.data,
@@ -248,4 +297,41 @@ extern "C" int cvchip_contract_twopoint(cvchip_ctx *c, double *contr, int idsour
   field_free(fc);
   field_free(fp);
   return rc;
+}
+
+// No counterpart in the reference.  cvchip_field_contract_twopoint for callers
+// that cannot allocate: the block partials go to the file-scope scratch array
+// (through its per-device address), the per-slice sums are formed on the device
+// and added to contr_dev, device memory of 2 T doubles that the caller owns.
+// Calls no hipMalloc, hipFree or allocating routine.  Calls that overlap on one
+// GPU share its scratch and race.
+extern "C" int cvchip_field_contract_twopoint_noalloc(cvchip_ctx *c, double *contr_dev, int idsource, int idsink,
+                                                      const cvchip_field *chi, const cvchip_field *phi, int n_c,
+                                                      const int snk_mom[3], const int src[4]) {
+  if (!c || !contr_dev || !chi || !phi || !src) {
+    set_error("NULL argument to cvchip_field_contract_twopoint_noalloc");
+    return CVCHIP_ERR_NULL;
+  }
+  const Lattice &L = c->lat;
+  const size_t need = 2 * (size_t)L.T * (size_t)((L.LX * L.LY * L.LZ + 255) / 256);
+  if (need > kTwoPtScratch) {
+    set_error("two-point function without allocation: %zu partial sums exceed the scratch of %zu "
+              "(T * ceil(LX LY LZ / 256) must stay at or below %zu)", need, kTwoPtScratch, kTwoPtScratch / 2);
+    return CVCHIP_ERR_ARG;
+  }
+  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) {
+    set_error("two-point scratch of device %d not set up (create the context on that device)", c->device);
+    return CVCHIP_ERR_STATE;
+  }
+  double *part = g_scratch_ptr[c->device];
+  TwoPt P;
+  size_t nblk = 0;
+  CVC_TRY(twopoint_params(c, idsource, idsink, chi->f, phi->f, n_c, snk_mom, src, P, nblk));
+  hipLaunchKernelGGL(k_twopoint, dim3((unsigned)nblk), dim3(256), 0, c->stream, (const double2 *)chi->f.data,
+                     (const double2 *)phi->f.data, P, part);
+  hipLaunchKernelGGL(k_twopoint_final, dim3(1), dim3(256), 0, c->stream, (const double *)part, contr_dev, L.T,
+                     P.nchunk);
+  CVC_HIP(hipGetLastError());
+  CVC_HIP(hipStreamSynchronize(c->stream));
+  return CVCHIP_OK;
 }

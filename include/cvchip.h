@@ -255,6 +255,16 @@ int cvchip_contract_twopoint(cvchip_ctx *ctx, double *contr, int idsource, int i
 int cvchip_field_contract_twopoint(cvchip_ctx *ctx, double *contr, int idsource, int idsink,
                                    const cvchip_field *chi, const cvchip_field *phi, int n_c,
                                    const int snk_mom[3], const int src[4]);
+/* No counterpart in the reference: cvchip_field_contract_twopoint for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * Adds C(t) to contr_dev, device memory of 2 T doubles owned by the caller,
+ * and allocates nothing: the per-block partial sums live in a file-scope
+ * device array of 262144 doubles (2 MiB) per GPU, so the lattice must satisfy
+ * T * ceil(LX LY LZ / 256) <= 131072 (else CVCHIP_ERR_ARG).  Calls that overlap
+ * on one GPU share that array and race. */
+int cvchip_field_contract_twopoint_noalloc(cvchip_ctx *ctx, double *contr_dev, int idsource, int idsink,
+                                           const cvchip_field *chi, const cvchip_field *phi, int n_c,
+                                           const int snk_mom[3], const int src[4]);
 
 /* ------------------------------------------------ LIME / SciDAC / ILDG files
  * Host-only readers and writers of the reference's file formats (no GPU).
