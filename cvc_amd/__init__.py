@@ -29,6 +29,7 @@ EXPORTS = [
     "cvchip_lime_write_propagator", "cvchip_cvc_propagators", "cvchip_field_point_sources", "cvchip_field_zero",
     "cvchip_field_copy_columns", "cvchip_gamma_tables", "cvchip_contract_twopoint",
     "cvchip_field_contract_twopoint", "cvchip_field_contract_twopoint_noalloc",
+    "cvchip_field_sequential_sources", "cvchip_p2gg_source",
     "cvchip_ctx_set_dslash_policy",
     "cvchip_last_error", "cvchip_set_gauge", "cvchip_Q_phi", "cvchip_scalar_product_co",
     "cvchip_scalar_product_re", "cvchip_invert_Qtm", "cvchip_invert_Qtm_her", "cvchip_field_create",
@@ -126,6 +127,10 @@ def lib():
                                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cvchip_field_contract_twopoint_noalloc.argtypes = [_vp, C.c_void_p, C.c_int, C.c_int, _vp, _vp, C.c_int,
                                                              C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cvchip_field_sequential_sources.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                                      C.POINTER(C.c_int), C.c_double]
+        L.cvchip_p2gg_source.argtypes = [_vp, _dp, _dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int),
+                                         C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -500,6 +505,22 @@ class Context:
         _check(lib().cvchip_cvc_propagators(self._h, ptrs, (C.c_int * 4)(*src), float(sp), int(niter_max), int(prec),
                                             float(delta), C.byref(st)))
         return props, st.as_dict()
+
+    def field_sequential_sources(self, out: Field, inp: Field, col0, tseq, gamma_id, mom=None, sgn=1.0):
+        _check(lib().cvchip_field_sequential_sources(self._h, out._h, inp._h, int(col0), int(tseq), int(gamma_id),
+                                                     (C.c_int * 3)(*mom) if mom is not None else None,
+                                                     float(sgn)))
+
+    def p2gg_source(self, src, seq_timeslice, seq_gamma_id, seq_mom=(0, 0, 0), propagator_position=0, sp=1e-20,
+                    niter_max=1000, prec=PREC_F64, delta=0.1):
+        """P -> gamma gamma for one source point: (conn[V,16,2], contact[4] complex, stats)."""
+        conn = np.empty((self.V, 16, 2))
+        contact = np.zeros(8)
+        st = Stats()
+        _check(lib().cvchip_p2gg_source(self._h, _ptr(conn), _ptr(contact), (C.c_int * 4)(*src), int(seq_timeslice),
+                                        int(seq_gamma_id), (C.c_int * 3)(*seq_mom), int(propagator_position),
+                                        float(sp), int(niter_max), int(prec), float(delta), C.byref(st)))
+        return conn, contact[0::2] + 1j * contact[1::2], st.as_dict()
 
     def cvc_source(self, src, sp, niter_max=1000, prec=PREC_F64, delta=0.1):
         conn = np.empty((self.V, 16, 2))

@@ -384,6 +384,15 @@ __global__ void k_put_columns(double2 *__restrict__ dst, const double2 *__restri
   dst[(size_t)cs * ND + col0 + r] = src[i];
 }
 
+// dst columns col0 .. col0 + 11 of an nd-column FP64 set = the 12 columns of src
+int put_columns(cvchip_ctx *c, Field &dst, const Field &src, int col0) {
+  const uint32_t V = (uint32_t)c->lat.V;
+  hipLaunchKernelGGL(k_put_columns, dim3((144u * V + 255) / 256), dim3(256), 0, c->stream, (double2 *)dst.data,
+                     (const double2 *)src.data, V, dst.nrhs, col0);
+  CVC_HIP(hipGetLastError());
+  return 0;
+}
+
 // ------------------------------------------------------------------ Pi(q)
 // One pass of a length-L DFT along a direction of stride `stride` (in sites):
 // out[.. n ..] = sum_x w[(n x) mod L] in[.. x ..],  w[k] = exp(-2 pi i k / L).

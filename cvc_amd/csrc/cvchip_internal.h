@@ -151,6 +151,9 @@ int ensure_stage(cvchip_ctx *c, size_t bytes);
 // device address of the allocation-free two-point entry point's scratch on
 // `device` (the current device), taken once per device
 int twopoint_scratch_init(int device);
+// row a of the gamma matrix id (0-3 gamma_mu, 4 identity, 5 gamma5, 6-9
+// gamma_mu gamma5, 10-15 sigma-type products): entry re[a] + i im[a] at col[a]
+void gamma_matrix_rows(int id, int col[4], int re[4], int im[4]);
 int ensure_red(cvchip_ctx *c, size_t bytes);
 int ensure_red2(cvchip_ctx *c, size_t bytes);
 // sites: 0 for the whole lattice, or the half lattice V/2 of the even/odd solver

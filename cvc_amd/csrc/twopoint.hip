@@ -68,6 +68,17 @@ struct TwoPt {
 
 }  // namespace
 
+// row a of gamma matrix id (reference index order): the entry re[a] + i im[a] at column col[a]
+void gamma_matrix_rows(int id, int col[4], int re[4], int im[4]) {
+  UMat g[16];
+  gamma_mats(g);
+  for (int a = 0; a < 4; a++) {
+    col[a] = g[id].col[a];
+    re[a] = g[id].re[a];
+    im[a] = g[id].im[a];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_twopoint(const double2 *__restrict__ chi, const double2 *__restrict__ phi,
                                                   TwoPt P, double *__restrict__ part) {
   const int t = blockIdx.x / P.nchunk, chunk = blockIdx.x - t * P.nchunk;

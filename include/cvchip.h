@@ -266,6 +266,25 @@ int cvchip_field_contract_twopoint_noalloc(cvchip_ctx *ctx, double *contr_dev, i
                                            const cvchip_field *chi, const cvchip_field *phi, int n_c,
                                            const int snk_mom[3], const int src[4]);
 
+/* P -> gamma gamma (p2gg_xspace.c).  Sequential sources of p2gg_xspace.c:517-548
+ * on device sets: out column r = the source of in column col0 + r,
+ * out(x) = delta_{x_0, tseq} e^{i sgn 2 pi (x p_x / LX + y p_y / LY + z p_z / LZ)}
+ * Gamma_gamma_id in(x) (mom NULL: p = 0; the reference's sgn is
+ * 1 - 2 g_propagator_position). */
+int cvchip_field_sequential_sources(cvchip_ctx *ctx, cvchip_field *out, const cvchip_field *in, int col0,
+                                    int tseq, int gamma_id, const int mom[3], double sgn);
+/* One P -> gamma gamma source point (p2gg_xspace.c:393-899): the 120 point
+ * propagators from y and y + nu (as cvchip_cvc_source), the 60 sequential
+ * sources of the flavour propagator_position (0 up, 1 down) on the time slice
+ * (y_0 + seq_timeslice) mod T with gamma seq_gamma_id and momentum seq_mom
+ * (p2gg_xspace.c:517-548), their solves with that flavour's operator
+ * (:572-581), and the conserved-vector-current contraction of the sequential
+ * propagators with the point propagators of the other flavour (:615-899):
+ * conn[V][16][2] (-1/4 normalised, 4 mu + nu) and the contact term. */
+int cvchip_p2gg_source(cvchip_ctx *ctx, double *conn, double contact[8], const int src[4], int seq_timeslice,
+                       int seq_gamma_id, const int seq_mom[3], int propagator_position, double sp, int niter_max,
+                       int prec, double delta, cvchip_stats *stats);
+
 /* ------------------------------------------------ LIME / SciDAC / ILDG files
  * Host-only readers and writers of the reference's file formats (no GPU).
  * checksum, when not NULL, receives the SciDAC checksum (suma, sumb) of the

@@ -64,6 +64,7 @@ def lib():
         L.orc_plaquette.argtypes = [P, dp]
         L.orc_plaquette.restype = C.c_double
         L.orc_gamma_tables.argtypes = [C.POINTER((C.c_int * 24) * 16), C.POINTER((C.c_int * 24) * 16)]
+        L.orc_seq_source.argtypes = [P, dp, dp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_double]
         _lib = L
     return _lib
 
@@ -178,6 +179,13 @@ class Lattice:
         lib().orc_contract_twopoint(C.byref(self._s), _dp(out), int(idsource), int(idsink), cp, pp, int(n_c), mom,
                                     (C.c_int * 4)(*src))
         return out[:, 0] + 1j * out[:, 1]
+
+    def seq_source(self, prop, tseq, gid, mom=(0, 0, 0), sgn=1.0):
+        """Sequential source of p2gg_xspace.c:517-548 from one propagator column [V,24]."""
+        out = np.empty((self.V, 24))
+        lib().orc_seq_source(C.byref(self._s), _dp(np.ascontiguousarray(prop, dtype=np.float64)), _dp(out),
+                             int(tseq), int(gid), (C.c_int * 3)(*[int(m) for m in mom]), C.c_double(sgn))
+        return out
 
     def set_threads(self, n):
         """Threads of the operator loop (the reference's OpenMP Q_phi_tbc, Q_phi.c:190-206)."""

@@ -625,3 +625,27 @@ void orc_contract_twopoint(const orc_lat *L, double *contr, int idsource, int id
     }
   }
 }
+
+/* Sequential source of p2gg_xspace.c:517-548 (serial build): out = 0 except on
+ * the time slice tseq, where out(x) = e^{i phase(x)} Gamma_gid prop(x) with
+ * phase = sgn 2 pi (x p_x / LX + y p_y / LY + z p_z / LZ); the reference's sgn
+ * is 1 - 2 g_propagator_position.  Loop order and operations as there:
+ * _fv_eq_gamma_ti_fv, then _fv_eq_fv_ti_co. */
+void orc_seq_source(const orc_lat *L, const double *prop, double *out, int tseq, int gid,
+                    const int mom[3], double sgn) {
+  double spinor1[24], w[2];
+  memset(out, 0, sizeof(double) * 24 * L->V);
+  const int x0 = tseq;
+  for (int x1 = 0; x1 < L->LX; x1++)
+    for (int x2 = 0; x2 < L->LY; x2++)
+      for (int x3 = 0; x3 < L->LZ; x3++) {
+        const long ix = lex(L, x0, x1, x2, x3);
+        const double phase = sgn * 2. * ORC_PI * (x1 * mom[0] / (double)L->LX + x2 * mom[1] / (double)L->LY +
+                                                 x3 * mom[2] / (double)L->LZ);
+        w[0] = cos(phase);
+        w[1] = sin(phase);
+        gamma_ti_fv(L, spinor1, gid, prop + 24 * ix);
+        fv_ti_co(out + 24 * ix, spinor1, w);
+      }
+}
+

@@ -100,6 +100,12 @@ void orc_contract_twopoint(const orc_lat *L, double *contr, int idsource, int id
                            const double *const *chi, const double *const *phi, int n_c,
                            const int *snk_mom, const int src[4]);
 
+/* Sequential source of p2gg_xspace.c:517-548: on time slice tseq,
+ * out(x) = e^{i sgn 2 pi (x p_x / LX + y p_y / LY + z p_z / LZ)} Gamma_gid prop(x),
+ * zero elsewhere (sgn = 1 - 2 g_propagator_position in the reference). */
+void orc_seq_source(const orc_lat *L, const double *prop, double *out, int tseq, int gid,
+                    const int mom[3], double sgn);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,3 +88,16 @@ def test_e2e_fixture_is_frozen(orc):
     for nu in range(4):
         want[8 ** (3 - nu), nu] -= ct8[nu]
     assert np.max(np.abs(w - want)) < 1e-12 * np.max(np.abs(conn8))
+
+
+def test_p2gg_fixture_is_frozen(orc):
+    """golden_p2gg.npz (tests/golden/make_golden_p2gg.py): case a recomputed bit for bit
+    from the oracle's pipeline."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_golden_p2gg import CASES, p2gg_oracle
+    import cvc_amd
+    gold = np.load(os.path.join(HERE, "golden", "golden_p2gg.npz"), allow_pickle=False)
+    g = cvc_amd.gauge_random(tuple(int(v) for v in gold["dims"]), 1, 0.3)
+    conn, ct = p2gg_oracle(g, CASES["a"], 1)
+    assert np.array_equal(conn, gold["conn_a"]) and np.array_equal(ct, gold["contact_a"])
