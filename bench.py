@@ -237,7 +237,24 @@ def main():
             ctx.field_contract_twopoint(5, 5, one, one, src=srcs[k])
         extra["twopoint_48rhs_s"] = round(time.perf_counter() - s0, 4)
         log(f"four pion correlators from the 48 propagators: {extra['twopoint_48rhs_s']}s")
-        del b48, x48, one
+        # the same 48 columns as four sequential 12-rhs solves (one per source position)
+        b12, x12 = ctx.field(12), ctx.field(12)
+        for eo, name in ((False, "4x12rhs_mixed_f32"), (True, "4x12rhs_eo_mixed_f32")):
+            tot, its = 0.0, []
+            for k in range(4):
+                b12.copy_columns(0, b48, 12 * k, 12)
+                x12.zero()
+                torch.cuda.synchronize()
+                s0 = time.perf_counter()
+                rc, st = ctx.invert_her(x12, b12, 1e-20, 4000, prec=cvc_amd.PREC_F32, delta=a.delta, check=False,
+                                        eo=eo)
+                torch.cuda.synchronize()
+                tot += time.perf_counter() - s0
+                its.append(st["iterations"])
+            extra[f"solve_{name}_s"] = round(tot, 3)
+            extra[f"solve_{name}_iters"] = its
+            log(f"solve {name} (same sources, one after the other): {tot:.2f}s iterations {its}")
+        del b48, x48, one, b12, x12
 
     if not a.no_solve and not a.no_source:
         # one source point: 60 up + 60 down propagators (10 batches of 12, even/odd
