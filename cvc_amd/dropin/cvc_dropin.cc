@@ -64,8 +64,10 @@ void die(int rc, const char *where) {
 }
 
 // Hash of every 64-bit word of the gauge field (four independent
-// multiply-xorshift lanes, folded at the end): any change of any link entry,
-// also a single bit, changes it with probability 1 - 2^-64.  In place edits of
+// multiply-xorshift lanes, folded at the end).  Every step is a bijection of a
+// lane's state, so a change of one word (one link entry, also a single bit)
+// always changes the hash; changes of several words are missed only by a
+// chance collision of the 64-bit value.  In place edits of
 // g_gauge_field (a local gauge transformation as in apply_gt_gauge,
 // cvc_utils.c:2044-2068, or a one-link finite difference) are seen by the next
 // operator or solver call.
