@@ -1,0 +1,75 @@
+"""Mixed-precision BiCGStab with FP64 reliable updates (GPU).
+
+A batch of point and Gaussian sources makes the columns' residual histories
+differ, so a column's own reliable-update trigger and the batch-wide update
+fall on different iterations.  A column forced into an update keeps the search
+direction it advanced in the x/p pass; advancing it a second time would corrupt
+its Krylov recurrence and cost iterations, so the FP32-inner solve must take
+about as many iterations as the FP64 solve of the same batch.
+
+A solve stopped by niter_max before any reliable update returns the FP64 x with
+the low-precision accumulator folded in (not the start vector): the true
+residual of that x, computed by the oracle's operator, is well below 1 and
+equals the reported true_res2_max.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+BC = (1.0, 0.0, 0.0, 0.0)
+
+
+@pytest.fixture(scope="module")
+def lat8(chip):
+    """8^4, gauge seed 1 (the plain-method configuration's field)."""
+    dims = (8, 8, 8, 8)
+    return dims, chip.gauge_random(dims, 1, 0.3)
+
+
+def mixed_batch(chip, V, seed):
+    rng = np.random.default_rng(seed)
+    pts = [chip.point_source(V, (97 * a) % V, a // 3, a % 3) for a in range(6)]
+    gauss = [rng.standard_normal((V, 24)) * 10.0 ** (-a) for a in range(6)]
+    return np.stack(pts + gauss)
+
+
+@pytest.mark.parametrize("eo", [False, True])
+def test_forced_reliable_updates_keep_iteration_count(chip, orc, lat8, eo):
+    dims, g = lat8
+    print(f"mixed FP32 vs FP64 iterations 8^4 eo={eo}", flush=True)
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=g)
+    b = mixed_batch(chip, ctx.V, 3)
+    rc64, x64, st64 = ctx.invert_Qtm_her(b, 1e-20, 5000, prec=chip.PREC_F64, eo=eo)
+    rc32, x32, st32 = ctx.invert_Qtm_her(b, 1e-20, 5000, prec=chip.PREC_F32, delta=0.01, eo=eo)
+    print(f"  FP64 {rc64} iterations; FP32 inner {rc32} iterations, "
+          f"{st32['reliable_updates']} reliable updates", flush=True)
+    assert st64["converged"] == 12 and st32["converged"] == 12
+    assert st32["true_res2_max"] <= 1e-20
+    assert st32["reliable_updates"] >= 2
+    # measured 122 / 114 (plain) and 39 / 42 (even/odd); advancing p twice gave 132 and 55
+    assert rc32 <= rc64 + max(4, int(0.12 * rc64)), (rc32, rc64)
+    L = orc.Lattice(dims, 0.163, 0.01, BC)
+    for a in (0, 6, 11):
+        res = L.Q_phi(g, x32[a]) - b[a]
+        assert L.dot_re(res, res) / L.dot_re(b[a], b[a]) < 1e-20
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_unconverged_mixed_solve_returns_its_residual(chip, orc, lat8, prec):
+    dims, g = lat8
+    print(f"mixed {prec} stopped by niter_max 8^4", flush=True)
+    P = chip.PREC_F32 if prec == "f32" else chip.PREC_F16
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=g)
+    b = mixed_batch(chip, ctx.V, 4)
+    # delta 1e-9: no replacement within 25 iterations, so everything the solve
+    # achieved is still in the low-precision accumulator when it stops
+    rc, x, st = ctx.invert_Qtm_her(b, 1e-24, 25, prec=P, delta=1e-9, check=False)
+    assert rc == chip.ERR_NOCONV and st["converged"] == 0 and st["reliable_updates"] == 0
+    L = orc.Lattice(dims, 0.163, 0.01, BC)
+    worst = 0.0
+    for a in range(12):
+        res = L.Q_phi(g, x[a]) - b[a]
+        worst = max(worst, L.dot_re(res, res) / L.dot_re(b[a], b[a]))
+    print(f"  reported {st['true_res2_max']:.3e}, oracle {worst:.3e}", flush=True)
+    assert 1e-24 < worst < 0.1   # x = 0 (the accumulator dropped) would give 1
+    assert abs(worst - st["true_res2_max"]) <= 1e-6 * worst
