@@ -490,8 +490,8 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
         // -1/2 hop + phi/(2 kappa) + i mutm gamma5 phi  (gamma5 = +1 on spins 0,1, -1 on 2,3)
         const R m = c < 6 ? P.mutm : -P.mutm;
         C o;
-        o.x = mh * acc[c].x + P.k2 * own[c].x - m * own[c].y;
-        o.y = mh * acc[c].y + P.k2 * own[c].y + m * own[c].x;
+        o.x = fmar(mh, acc[c].x, fmar(P.k2, own[c].x, -(m * own[c].y)));
+        o.y = fmar(mh, acc[c].y, fmar(P.k2, own[c].y, m * own[c].x));
         if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
         acc[c] = o;
       }
@@ -500,19 +500,19 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
 #pragma unroll
       for (int c = 0; c < 12; c++) {
         const R m = c < 6 ? P.mutm : -P.mutm;
-        C h = mk<R>(P.hopc * acc[c].x, P.hopc * acc[c].y);
         C o;
         if (P.dmode == 0) {
-          o.x = h.x + P.k2 * own[c].x - m * own[c].y;
-          o.y = h.y + P.k2 * own[c].y + m * own[c].x;
+          o.x = fmar(P.hopc, acc[c].x, fmar(P.k2, own[c].x, -(m * own[c].y)));
+          o.y = fmar(P.hopc, acc[c].y, fmar(P.k2, own[c].y, m * own[c].x));
         } else {
-          if (P.dmode >= 2) h = cadd(h, own[c]);
+          C h = mk<R>(P.hopc * acc[c].x, P.hopc * acc[c].y);
+          if (P.dmode >= 2) h = mk<R>(fmar(P.hopc, acc[c].x, own[c].x), fmar(P.hopc, acc[c].y, own[c].y));
           if (P.dmode == 2) {
             o = h;
           } else {
             // A^-1 h = (1/(2 kappa) - i mutm gamma5) h / ((1/(2 kappa))^2 + mutm^2)
-            o.x = (P.k2 * h.x + m * h.y) * P.dinv;
-            o.y = (P.k2 * h.y - m * h.x) * P.dinv;
+            o.x = fmar(P.k2, h.x, m * h.y) * P.dinv;
+            o.y = fmar(P.k2, h.y, -(m * h.x)) * P.dinv;
           }
         }
         if (G5OUT && c >= 6) { o.x = -o.x; o.y = -o.y; }
@@ -743,8 +743,8 @@ k_dslash_march(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, 
 #pragma unroll
       for (int c = 0; c < 12; c++) {
         const R m = c < 6 ? P.mutm : -P.mutm;
-        acc[c].x = (R)-2 * (P.k2 * own[c].x - m * own[c].y);
-        acc[c].y = (R)-2 * (P.k2 * own[c].y + m * own[c].x);
+        acc[c].x = (R)-2 * fmar(P.k2, own[c].x, -(m * own[c].y));
+        acc[c].y = (R)-2 * fmar(P.k2, own[c].y, m * own[c].x);
       }
       // backward t hop from the carry, then the carry for slice t+1 from phi(t)
       expand_w<C, 0, false>(acc, carry);

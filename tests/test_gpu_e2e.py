@@ -5,8 +5,9 @@ contracts them; the expected conn, contact term and Pi(q) come from the
 oracle's own 120 solves (the CPU restatement of invert_Qtm.c:276-478) fed to
 the oracle contraction (cvc_exact2_xspace.c:577-861), frozen in
 tests/golden/golden_e2e.npz (tests/golden/make_golden_e2e.py).  Both sides
-solve to a squared relative residual of 1e-28, which puts the solvers' share
-of the contraction error at the rounding level (the oracle's conn from solves
+solve to a squared relative residual of 1e-28 (the FP32-inner solves to 1e-26,
+see below), which puts the solvers' share of the contraction error at the
+rounding level (the oracle's conn from solves
 at 1e-28 and 1e-30 differ by 4.5e-15 relative at 4^4), so the comparison
 tests the whole chain at the 1e-10 bar:
 
@@ -89,6 +90,11 @@ def test_whole_source_mixed_and_eo_match_oracle_solves(chip, gold, prec, eo):
     if eo:
         ctx.set_solver_flags(chip.SOLVE_EO)
     p = chip.PREC_F32 if prec == "f32" else chip.PREC_F64
-    conn, ct, st = ctx.cvc_source(src, float(gold["sp"]), 20000, prec=p, delta=0.01)
-    assert st["converged"] == 120 and st["true_res2_max"] < 1e-26, st
+    # the mixed solvers decide convergence on the FP64 true residual, whose floor
+    # on this near-critical system is about 1e-28 (the oracle's own solves to
+    # 1e-28 end at true residuals up to 5e-28): ask them for 1e-26, which moves
+    # the propagators by about 1e-13 relative, far inside the 1e-10 bar
+    sp = max(float(gold["sp"]), 1e-26) if prec == "f32" else float(gold["sp"])
+    conn, ct, st = ctx.cvc_source(src, sp, 20000, prec=p, delta=0.01)
+    assert st["converged"] == 120 and st["true_res2_max"] <= 1e-26, st
     check(ctx, conn, ct, gold, key)
