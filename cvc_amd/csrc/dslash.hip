@@ -22,6 +22,11 @@ template <typename R>
 struct DParams {
   uint32_t V, VN, sX, sT;
   int N, T, LX, LY, LZ;
+  // log2 of LZ, LY, LX, sX, sT and of the traversal's LZt when a power of two,
+  // else -1 (coordinates then come from shifts and masks instead of divisions);
+  // 1/N, 1/S and 1/(S KL) for the small quotients of thread and staging indices
+  int shLZ, shLY, shLX, shSX, shST, shLZt;
+  float invN, invS, invKS;
   int partner;   // DOT_TS: dot with the separate field r2 instead of the input
   // traversal (mode 1): the lattice is cut into chunks of csz consecutive
   // sites (one eighth of a time slice when possible), bpc blocks per chunk;
@@ -68,6 +73,15 @@ struct DParams {
   const int *done;
   int dstride;
 };
+
+// n / d and n % d for a divisor fixed for the launch: a shift and a mask when
+// d = 2^sh (sh >= 0, a branch uniform across the grid)
+__device__ __forceinline__ uint32_t qdiv(uint32_t n, uint32_t d, int sh) { return sh >= 0 ? n >> sh : n / d; }
+__device__ __forceinline__ uint32_t qmod(uint32_t n, uint32_t d, int sh) { return sh >= 0 ? n & (d - 1u) : n % d; }
+// floor(n / d) for n < 2^21 from inv, the float nearest 1/d (or within two
+// roundings of it): (n + 1/2) / d stays at least 1/(2d) away from an integer,
+// and the float product errs by less than (n + 1/2) 2^-22 / d
+__device__ __forceinline__ uint32_t sdiv(uint32_t n, float inv) { return (uint32_t)(((float)n + 0.5f) * inv); }
 
 // Spin projection of a neighbour spinor: h[a] = psi_a -+ g_a psi_{GCOL[a]}
 // for the upper spins a = 0, 1 ((1 -+ gamma_mu) has rank 2 in the chiral
@@ -313,11 +327,11 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     if constexpr (!EO) {
       return v;
     } else {
-      const uint32_t zl = v % P.LZt;
-      uint32_t q = v / P.LZt;
-      const uint32_t y = q % (uint32_t)P.LY;
-      q /= (uint32_t)P.LY;
-      const uint32_t x = q % (uint32_t)P.LX, t = q / (uint32_t)P.LX;
+      const uint32_t zl = qmod(v, P.LZt, P.shLZt);
+      uint32_t q = qdiv(v, P.LZt, P.shLZt);
+      const uint32_t y = qmod(q, (uint32_t)P.LY, P.shLY);
+      q = qdiv(q, (uint32_t)P.LY, P.shLY);
+      const uint32_t x = qmod(q, (uint32_t)P.LX, P.shLX), t = qdiv(q, (uint32_t)P.LX, P.shLX);
       return ((q * (uint32_t)P.LY + y) * (uint32_t)P.LZ) + 2u * zl + (((uint32_t)P.par + t + x + y) & 1u);
     }
   };
@@ -381,8 +395,8 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     v = s0 + sl;
     return v < s_end;
   };
-  const uint32_t r = threadIdx.x % (uint32_t)P.N;
-  const uint32_t sl = threadIdx.x / (uint32_t)P.N;
+  const uint32_t sl = sdiv(threadIdx.x, P.invN);
+  const uint32_t r = threadIdx.x - sl * (uint32_t)P.N;
   uint32_t vs;
   const bool active = slot_v(sl, vs) && !(P.done && P.done[(size_t)r * P.dstride]);
   const uint32_t i = vs * (uint32_t)P.N + r;
@@ -399,13 +413,15 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
     if (hopi & 1u) return site;
     const int mu = (int)(hopi >> 1);
     const uint32_t str = mu == 0 ? P.sT : mu == 1 ? P.sX : mu == 2 ? (uint32_t)P.LZ : 1u;
+    const int shs = mu == 0 ? P.shST : mu == 1 ? P.shSX : mu == 2 ? P.shLZ : 0;
     const uint32_t L = (uint32_t)(mu == 0 ? P.T : mu == 1 ? P.LX : mu == 2 ? P.LY : P.LZ);
-    const uint32_t crd = (site / str) % L;
+    const int shl = mu == 0 ? -1 : mu == 1 ? P.shLX : mu == 2 ? P.shLY : P.shLZ;
+    const uint32_t crd = qmod(qdiv(site, str, shs), L, shl);
     return crd == 0 ? site + (L - 1) * str : site - str;
   };
   if constexpr (FOLD) {
     for (uint32_t e = threadIdx.x; e < 8u * S; e += blockDim.x) {
-      const uint32_t hopi = e / S, sle = e - hopi * S;
+      const uint32_t hopi = sdiv(e, P.invS), sle = e - hopi * S;
       uint32_t ve;
       if (!slot_v(sle, ve)) continue;
       const uint32_t ls = link_site(hopi, lex_of(ve));
@@ -421,8 +437,8 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
   } else if constexpr (LDSL) {
     const uint32_t tot = 8u * KL * S;
     for (uint32_t e = threadIdx.x; e < tot; e += blockDim.x) {
-      const uint32_t hopi = e / (KL * S), rem = e - hopi * (KL * S);
-      const uint32_t k = rem / S, sle = rem - k * S;
+      const uint32_t hopi = sdiv(e, P.invKS), rem = e - hopi * (KL * S);
+      const uint32_t k = sdiv(rem, P.invS), sle = rem - k * S;
       uint32_t ve;
       if (!slot_v(sle, ve)) continue;
       const uint32_t site = lex_of(ve);
@@ -430,8 +446,10 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
       uint32_t ls = site;
       if ((hopi & 1u) == 0) {          // backward hop: link at site - mu
         const uint32_t str = mu == 0 ? P.sT : mu == 1 ? P.sX : mu == 2 ? (uint32_t)P.LZ : 1u;
+        const int shs = mu == 0 ? P.shST : mu == 1 ? P.shSX : mu == 2 ? P.shLZ : 0;
         const uint32_t L = (uint32_t)(mu == 0 ? P.T : mu == 1 ? P.LX : mu == 2 ? P.LY : P.LZ);
-        const uint32_t crd = (site / str) % L;
+        const int shl = mu == 0 ? -1 : mu == 1 ? P.shLX : mu == 2 ? P.shLY : P.shLZ;
+        const uint32_t crd = qmod(qdiv(site, str, shs), L, shl);
         ls = crd == 0 ? site + (L - 1) * str : site - str;
       }
       Ul[(hopi * KL + k) * LDS_SITES + sle] = U[((size_t)mu * KL + k) * P.V + ls];
@@ -456,11 +474,11 @@ k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict_
   const NbLoader<IOI, LDSS> ld{in, Sp, (uint32_t)P.N, r, s0, S};
   if (active) {
     const uint32_t s = lex_of(vs);
-    const uint32_t t = s / P.sT;
+    const uint32_t t = qdiv(s, P.sT, P.shST);
     uint32_t rem = s - t * P.sT;
-    const uint32_t x = rem / P.sX;
+    const uint32_t x = qdiv(rem, P.sX, P.shSX);
     rem -= x * P.sX;
-    const uint32_t y = rem / (uint32_t)P.LZ;
+    const uint32_t y = qdiv(rem, (uint32_t)P.LZ, P.shLZ);
     const uint32_t z = rem - y * (uint32_t)P.LZ;
 
     C acc[12];
@@ -832,6 +850,13 @@ k_dslash_march(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, 
 
 // ------------------------------------------------------------- launcher
 
+static int log2_exact(uint32_t v) {   // -1 unless v is a power of two
+  if (v == 0 || (v & (v - 1))) return -1;
+  int l = 0;
+  while ((1u << l) < v) l++;
+  return l;
+}
+
 template <typename R>
 static DParams<R> make_params(const cvchip_ctx *c, int N, double mutm) {
   DParams<R> P;
@@ -844,6 +869,14 @@ static DParams<R> make_params(const cvchip_ctx *c, int N, double mutm) {
   P.LZ = c->lat.LZ;
   P.sX = (uint32_t)(c->lat.LY * c->lat.LZ);
   P.sT = (uint32_t)(c->lat.LX * c->lat.LY * c->lat.LZ);
+  P.shLZ = log2_exact((uint32_t)c->lat.LZ);
+  P.shLY = log2_exact((uint32_t)c->lat.LY);
+  P.shLX = log2_exact((uint32_t)c->lat.LX);
+  P.shSX = log2_exact(P.sX);
+  P.shST = log2_exact(P.sT);
+  P.shLZt = P.shLZ;
+  P.invN = 1.0f / (float)N;
+  P.invS = P.invKS = 0.f;
   P.k2 = (R)(0.5 / c->kappa);
   P.mutm = (R)mutm;
   for (int m = 0; m < 4; m++) P.ph[m] = mk<R>((R)c->ph.re[m], (R)c->ph.im[m]);
@@ -905,12 +938,6 @@ static bool march_geometry(const cvchip_ctx *c, int N, Traversal &t, bool explic
   return true;
 }
 
-static int log2_exact(uint32_t v) {   // -1 unless v is a power of two
-  if (v == 0 || (v & (v - 1))) return -1;
-  int l = 0;
-  while ((1u << l) < v) l++;
-  return l;
-}
 
 // Block-to-site mapping; the policy's bits 4-7 pick the mode (0: automatic),
 // bits 8-15 the y-block height of modes 2, 4 and 5 (0: automatic), bits 16-19
@@ -1042,11 +1069,17 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   DParams<R> P = make_params<R>(c, N, mutm);
   P.partner = (dm == DOT_TS && r2 != nullptr) ? 1 : 0;
   P.LZt = EO ? (uint32_t)c->lat.LZ / 2 : (uint32_t)c->lat.LZ;
+  P.shLZt = log2_exact(P.LZt);
   P.par = eo ? eo->par : 0;
   P.dmode = eo ? eo->dmode : 0;
   P.hopc = (R)(eo ? eo->hopc : -0.5);
   P.dinv = (R)(1.0 / (0.25 / (c->kappa * c->kappa) + mutm * mutm));
   const int B = block_for(N);
+  {
+    const int S = B / N, KLs = RECON == 18 ? 9 : 6;   // sites per block, link entries per hop
+    P.invS = 1.0f / (float)S;
+    P.invKS = 1.0f / (float)(KLs * S);
+  }
   const Traversal tr = traversal(c, N, EO, IOI::K == 2);
   P.csz = tr.csz;
   P.bpc = tr.bpc;
