@@ -37,7 +37,10 @@ struct RhsState {
   double normb, norm, tol2;
   double rmax;        // mixed: |r|^2 at the last reliable update
   double true_res2;
+  double best;        // mixed: the smallest true |r|^2 at a reliable update so far
   int done, final_upd, iter, upd, last_upd;
+  int stall;          // mixed: own reliable updates in a row that did not halve best
+  int stalled;        // mixed: stopped at the floor of the true residual, not converged
 };
 
 // ------------------------------------------------------------ vector kernels
@@ -259,6 +262,9 @@ __global__ void k_sc_init(RhsState *st, const double *res, int N, double sp, int
   S.r0rn[0] = S.norm;
   S.r0rn[1] = 0.;
   S.rmax = S.norm;
+  S.best = S.norm;
+  S.stall = 0;
+  S.stalled = 0;
   S.final_upd = 0;
   S.upd = 0;
   S.last_upd = 0;
@@ -339,6 +345,14 @@ __global__ void k_sc_beta(RhsState *st, const double *res, int N, int it, int la
   S.r0rn[1] = w[1];
 }
 
+// A column whose own reliable-update trigger fired kMaxStall times in a row
+// without the true |r|^2 falling below half its best value has reached the
+// rounding floor of the FP64 residual b - D x (a tolerance below that floor is
+// never met; at the floor the true residual only fluctuates): it stops, not
+// converged, with x as folded in at that update.  Updates forced on a column by
+// another column's trigger do not count.
+constexpr int kMaxStall = 6;
+
 // mixed: after the FP64 residual recompute (res: <r,r>, unused) decide convergence
 __global__ void k_sc_reliable(RhsState *st, const double *res, int N, int it) {
   const int r = threadIdx.x;
@@ -350,6 +364,15 @@ __global__ void k_sc_reliable(RhsState *st, const double *res, int N, int it) {
   S.last_upd = it;
   if (S.norm <= S.tol2) {
     S.done = 1;
+    S.iter = it;
+    return;
+  }
+  if (S.norm < 0.5 * S.best) {
+    S.best = S.norm;
+    S.stall = 0;
+  } else if (S.upd == 1 && ++S.stall >= kMaxStall) {
+    S.done = 1;
+    S.stalled = 1;
     S.iter = it;
   }
 }
@@ -831,11 +854,13 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
     }
   }
   TRY(pull_state(S));
-  int nconv = 0, maxit = 0;
+  int nconv = 0, maxit = 0, nstall = 0;
   for (int r = 0; r < N; r++)
-    if (S.hst[r].done) {
+    if (S.hst[r].done && !S.hst[r].stalled) {
       nconv++;
       if (S.hst[r].iter > maxit) maxit = S.hst[r].iter;
+    } else if (S.hst[r].stalled) {
+      nstall++;
     }
   // fold in what the low-precision accumulator still holds (non-converged rhs;
   // converged ones were folded at their last reliable update)
@@ -861,8 +886,13 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
   cleanup();
 #undef TRY
   if (nconv < N) {
-    set_error("mixed BiCGStab: %d of %d right-hand sides not converged after %d iterations", N - nconv, N,
-              niter_max + 1);
+    if (nstall)
+      set_error("mixed BiCGStab: %d of %d right-hand sides not converged (%d stopped at the floor of the FP64 true "
+                "residual, %d reliable updates without progress) after %d iterations", N - nconv, N, nstall,
+                kMaxStall, iters);
+    else
+      set_error("mixed BiCGStab: %d of %d right-hand sides not converged after %d iterations", N - nconv, N,
+                niter_max + 1);
     return CVCHIP_ERR_NOCONV;
   }
   return maxit;

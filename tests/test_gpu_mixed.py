@@ -96,3 +96,24 @@ def test_unconverged_mixed_solve_returns_its_residual(chip, orc, lat8, prec):
     print(f"  reported {st['true_res2_max']:.3e}, oracle {worst:.3e}", flush=True)
     assert 1e-24 < worst < 0.1   # x = 0 (the accumulator dropped) would give 1
     assert abs(worst - st["true_res2_max"]) <= 1e-6 * worst
+
+
+def test_unattainable_tolerance_stops_at_the_floor(chip, orc, lat8):
+    """A tolerance below the rounding floor of the FP64 true residual b - D x
+    (about 1e-28 on this near-critical system) ends the FP32-inner solve with
+    -3 after a few of its own reliable updates bring no progress, not after
+    niter_max iterations; the returned x is the solution at the floor."""
+    dims, g = lat8
+    print("mixed FP32 solve to an unattainable tolerance 8^4", flush=True)
+    ctx = chip.Context(dims, 0.163, 0.01, BC, gauge=g)
+    b = mixed_batch(chip, ctx.V, 3)[:6]
+    rc, x, st = ctx.invert_Qtm_her(b, 1e-34, 20000, prec=chip.PREC_F32, delta=0.01, check=False)
+    msg = chip.last_error()
+    print(f"  rc {rc}, {st['iterations']} iterations, {st['reliable_updates']} reliable updates: {msg}", flush=True)
+    assert rc == chip.ERR_NOCONV and st["converged"] == 0
+    assert "floor" in msg
+    assert st["iterations"] < 2000
+    L = orc.Lattice(dims, 0.163, 0.01, BC)
+    for a in range(6):
+        res = L.Q_phi(g, x[a]) - b[a]
+        assert L.dot_re(res, res) / L.dot_re(b[a], b[a]) < 1e-24
