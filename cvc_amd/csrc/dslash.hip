@@ -14,6 +14,8 @@
 // halo.  Optional epilogues fuse the solver's per-rhs dot products.
 #include <algorithm>
 
+#include <type_traits>
+
 #include "spinor_io.h"
 
 namespace cvchip {
@@ -314,10 +316,12 @@ template <typename R> struct DslashWaves {
 // LDSL: the block's sites share their links through LDS (N >= 8, at most
 // LDS_SITES sites per block): every link entry crosses the texture path once
 // per block instead of once per rhs lane.
-template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL, bool LDSS, bool EO>
+// IOR: the storage of r2, the shadow residual of the <r2, out> epilogue (the
+// FP32 mixed solver keeps it in FP16 with a per-site scale)
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, bool LDSL, bool LDSS, bool EO, typename IOR = IOI>
 __global__ void __launch_bounds__(256, DslashWaves<typename IOI::real>::value)
 k_dslash(IOO out, IOI in, IOI own_f, const cplx<typename IOI::real> *__restrict__ U,
-         DParams<typename IOI::real> P, IOI r2, double *__restrict__ part) {
+         DParams<typename IOI::real> P, IOR r2, double *__restrict__ part) {
   using R = typename IOI::real;
   using C = cplx<R>;
   constexpr int KL = RECON == 18 ? 9 : 6;
@@ -666,10 +670,10 @@ template <typename R> struct MarchWaves {
   static constexpr int value = sizeof(R) == 8 ? CVC_MARCH_WAVES64 : CVC_MARCH_WAVES32;
 };
 
-template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM>
+template <typename IOI, typename IOO, int RECON, bool G5OUT, int DM, typename IOR = IOI>
 __global__ void __launch_bounds__(256, MarchWaves<typename IOI::real>::value)
 k_dslash_march(IOO out, IOI in, const cplx<typename IOI::real> *__restrict__ U, DParams<typename IOI::real> P,
-               IOI r2, double *__restrict__ part) {
+               IOR r2, double *__restrict__ part) {
   using R = typename IOI::real;
   using C = cplx<R>;
   constexpr int KL = RECON == 18 ? 9 : 6;
@@ -1061,9 +1065,15 @@ struct EOArgs {
   double hopc;
 };
 
-template <typename IOI, typename IOO, int RECON, bool EO>
+// IOR differs from IOI only for the <r2, out> epilogue (DOT_R2) with an FP16 r2
+template <typename IOI, typename IOO, int RECON, bool EO, typename IOR = IOI>
 static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g5, DotMode dm,
                   const Field *r2, int *nblk_out, const EOArgs *eo = nullptr) {
+  constexpr bool kSameR = std::is_same<IOR, IOI>::value;
+  if (!kSameR && (dm != DOT_R2 || !r2)) {
+    set_error("operator: a shadow residual of another precision serves only the <r2, out> epilogue");
+    return CVCHIP_ERR_ARG;
+  }
   using R = typename IOI::real;
   const int N = in.nrhs;
   DParams<R> P = make_params<R>(c, N, mutm);
@@ -1126,7 +1136,9 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
   else U = c->gauge32r12;
   IOO o = make_io<IOO>(out);
   IOI ii = make_io<IOI>(in);
-  IOI rr = r2 ? make_io<IOI>(*r2) : ii;
+  IOR rr;
+  if constexpr (kSameR) rr = r2 ? make_io<IOI>(*r2) : ii;
+  else rr = make_io<IOR>(*r2);
   IOI ow = (eo && eo->own) ? make_io<IOI>(*eo->own) : ii;
   dim3 g(nblk), b(B);
   if (!EO && tr.mode == 6) {
@@ -1136,8 +1148,11 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
     const size_t lds = (size_t)kMarchSlots * KSM * tr.mS * sizeof(CR) + (size_t)((c->dslash_policy >> 24) & 15) * 8192;
     dim3 bm(tr.threads);
 #define CVC_MLAUNCH(G5, DMV) \
-    hipLaunchKernelGGL((k_dslash_march<IOI, IOO, RECON, G5, DMV>), g, bm, lds, c->stream, o, ii, U, P, rr, part)
-    if (g5) {
+    hipLaunchKernelGGL((k_dslash_march<IOI, IOO, RECON, G5, DMV, IOR>), g, bm, lds, c->stream, o, ii, U, P, rr, part)
+    if constexpr (!kSameR) {
+      if (g5) CVC_MLAUNCH(true, DOT_R2);
+      else CVC_MLAUNCH(false, DOT_R2);
+    } else if (g5) {
       if (dm == DOT_NONE) CVC_MLAUNCH(true, DOT_NONE);
       else if (dm == DOT_TS) CVC_MLAUNCH(true, DOT_TS);
       else if (dm == DOT_NN) CVC_MLAUNCH(true, DOT_NN);
@@ -1162,16 +1177,19 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
 #define CVC_LAUNCH(G5, DMV)                                                                                     \
   do {                                                                                                          \
     if constexpr (EO) {                                                                                         \
-      if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false, true>), g, b, dyn, c->stream, o, ii, ow, U, P, rr, part); \
-      else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false, true>), g, b, dyn, c->stream, o, ii, ow, U, P, rr, part); \
+      if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false, true, IOR>), g, b, dyn, c->stream, o, ii, ow, U, P, rr, part); \
+      else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false, true, IOR>), g, b, dyn, c->stream, o, ii, ow, U, P, rr, part); \
     } else {                                                                                                    \
-      if (ldsl && ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, true, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
-      else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
-      else if (ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, true, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
-      else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false, false>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      if (ldsl && ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, true, false, IOR>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      else if (ldsl) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, true, false, false, IOR>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      else if (ldss) hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, true, false, IOR>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
+      else hipLaunchKernelGGL((k_dslash<IOI, IOO, RECON, G5, DMV, false, false, false, IOR>), g, b, dyn, c->stream, o, ii, ii, U, P, rr, part); \
     }                                                                                                           \
   } while (0)
-  if (g5) {
+  if constexpr (!kSameR) {
+    if (g5) CVC_LAUNCH(true, DOT_R2);
+    else CVC_LAUNCH(false, DOT_R2);
+  } else if (g5) {
     if (dm == DOT_NONE) CVC_LAUNCH(true, DOT_NONE);
     else if (dm == DOT_TS) CVC_LAUNCH(true, DOT_TS);
     else if (dm == DOT_NN) CVC_LAUNCH(true, DOT_NN);
@@ -1194,7 +1212,10 @@ int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_o
     set_error("gauge field not set (call cvchip_set_gauge first)");
     return CVCHIP_ERR_STATE;
   }
-  if (out.nrhs != in.nrhs || out.V != in.V || (r2 && (r2->nrhs != in.nrhs || r2->prec != in.prec))) {
+  // r2 may be FP16 for FP32 fields in the <r2, out> epilogue (the FP32 mixed solver's shadow residual)
+  const bool r2_half = r2 && dm == DOT_R2 && in.prec == CVCHIP_PREC_F32 && r2->prec == CVCHIP_PREC_F16;
+  if (out.nrhs != in.nrhs || out.V != in.V ||
+      (r2 && (r2->nrhs != in.nrhs || r2->V != in.V || (r2->prec != in.prec && !r2_half)))) {
     set_error("operator fields do not match (nrhs %d vs %d)", out.nrhs, in.nrhs);
     return CVCHIP_ERR_ARG;
   }
@@ -1207,8 +1228,12 @@ int dslash(cvchip_ctx *c, Field &out, const Field &in, double mutm, int gamma5_o
       return launch<IOPlain<double>, IOPlain<double>, 12, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
     return launch<IOPlain<double>, IOPlain<double>, 18, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
   }
-  if (in.prec == CVCHIP_PREC_F32 && out.prec == CVCHIP_PREC_F32)
+  if (in.prec == CVCHIP_PREC_F32 && out.prec == CVCHIP_PREC_F32) {
+    if (r2_half)
+      return launch<IOPlain<float>, IOPlain<float>, 12, false, IOHalf>(c, out, in, mutm, gamma5_out, dm, r2,
+                                                                       nblocks_out);
     return launch<IOPlain<float>, IOPlain<float>, 12, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
+  }
   if (in.prec == CVCHIP_PREC_F16 && out.prec == CVCHIP_PREC_F16)
     return launch<IOHalf, IOHalf, 12, false>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out);
   set_error("unsupported operator precision pair %d -> %d", in.prec, out.prec);
@@ -1231,9 +1256,10 @@ int dslash_eo(cvchip_ctx *c, Field &out, const Field &in, const Field *own, doub
     return CVCHIP_ERR_ARG;
   }
   const int64_t Vh = c->lat.V / 2;
+  const bool r2_half = r2 && dm == DOT_R2 && in.prec == CVCHIP_PREC_F32 && r2->prec == CVCHIP_PREC_F16;
   if (out.V != Vh || in.V != Vh || out.nrhs != in.nrhs || out.prec != in.prec ||
       (dmode != 1 && (!own || own->V != Vh || own->nrhs != in.nrhs || own->prec != in.prec)) ||
-      (r2 && (r2->V != Vh || r2->nrhs != in.nrhs || r2->prec != in.prec)) || dmode < 0 || dmode > 3 ||
+      (r2 && (r2->V != Vh || r2->nrhs != in.nrhs || (r2->prec != in.prec && !r2_half))) || dmode < 0 || dmode > 3 ||
       (par != 0 && par != 1)) {
     set_error("even/odd operator: fields must be half-lattice sets of one size and precision");
     return CVCHIP_ERR_ARG;
@@ -1248,8 +1274,12 @@ int dslash_eo(cvchip_ctx *c, Field &out, const Field &in, const Field *own, doub
       return launch<IOPlain<double>, IOPlain<double>, 12, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
     return launch<IOPlain<double>, IOPlain<double>, 18, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
   }
-  if (in.prec == CVCHIP_PREC_F32)
+  if (in.prec == CVCHIP_PREC_F32) {
+    if (r2_half)
+      return launch<IOPlain<float>, IOPlain<float>, 12, true, IOHalf>(c, out, in, mutm, gamma5_out, dm, r2,
+                                                                      nblocks_out, &eo);
     return launch<IOPlain<float>, IOPlain<float>, 12, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
+  }
   return launch<IOHalf, IOHalf, 12, true>(c, out, in, mutm, gamma5_out, dm, r2, nblocks_out, &eo);
 }
 

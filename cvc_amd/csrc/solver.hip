@@ -23,6 +23,7 @@
 // previous such update (reliable updates), accumulating the low-precision
 // solution into the FP64 x at each update.  Convergence is always decided on
 // the FP64 true residual.
+#include <type_traits>
 #include <stdio.h>
 #include <algorithm>
 #include <chrono>
@@ -89,9 +90,9 @@ __global__ void __launch_bounds__(256) k_s_update(IO s, IO r1, IO p2, const RhsS
   vstore(s, vr, i, nt);
 }
 
-// r1 = s - omega t ; partial dots <r1,r1>, <r2,r1>
-template <typename IO>
-__global__ void __launch_bounds__(256) k_r1_update(IO r1, IO s, IO t, IO r2, const RhsState *__restrict__ st,
+// r1 = s - omega t ; partial dots <r1,r1>, <r2,r1>  (r2 stored as IOR)
+template <typename IO, typename IOR = IO>
+__global__ void __launch_bounds__(256) k_r1_update(IO r1, IO s, IO t, IOR r2, const RhsState *__restrict__ st,
                                                    uint32_t VN, int N, double *__restrict__ part, int nt) {
   CVC_ELEMWISE_PROLOGUE
   using R = typename IO::real;
@@ -160,9 +161,9 @@ __global__ void __launch_bounds__(256) k_xp_update(IOX x, IO p, IO s, IO r1, IO 
   }
 }
 
-// r = b - t  (t = D x), optionally copied to r2 and p; partial dots <r,r>, <b,b>
-template <typename IOB, typename IO>
-__global__ void __launch_bounds__(256) k_residual(IO r, IO r2, IO p, IOB b, IOB t, int copies,
+// r = b - t  (t = D x), optionally copied to r2 (stored as IOR) and p; partial dots <r,r>, <b,b>
+template <typename IOB, typename IO, typename IOR = IO>
+__global__ void __launch_bounds__(256) k_residual(IO r, IOR r2, IO p, IOB b, IOB t, int copies,
                                                   const RhsState *__restrict__ st, int only_upd,
                                                   uint32_t VN, int N, double *__restrict__ part) {
   CVC_ELEMWISE_PROLOGUE
@@ -231,12 +232,13 @@ __global__ void __launch_bounds__(256) k_accum_zero(IOX x, IOL xl, const RhsStat
 }
 
 // <a,b> complex and <a,a>: partials for Q=3
-template <typename IO>
-__global__ void __launch_bounds__(256) k_dot3(IO a, IO b, uint32_t VN, int N, double *__restrict__ part) {
+template <typename IOA, typename IOB = IOA>
+__global__ void __launch_bounds__(256) k_dot3(IOA a, IOB b, uint32_t VN, int N, double *__restrict__ part) {
   CVC_ELEMWISE_PROLOGUE
   double dv[3] = {0., 0., 0.};
   if (act) {
-    cplx<typename IO::real> va[12], vb[12];
+    cplx<typename IOA::real> va[12];
+    cplx<typename IOB::real> vb[12];
     a.load(va, i);
     b.load(vb, i);
 #pragma unroll
@@ -392,6 +394,17 @@ __global__ void k_sc_r0rn(RhsState *st, const double *res, int N) {
   S.beta[1] = w2[0] * w3[1] + w2[1] * w3[0];
   S.r0rn[0] = w[0];
   S.r0rn[1] = w[1];
+}
+
+// r0rn = <r2, r> (res: complex) for the running rhs, at the start of a solve
+// whose shadow residual r2 is stored in lower precision than r
+__global__ void k_sc_set_r0rn(RhsState *st, const double *res, int N) {
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  RhsState &S = st[r];
+  if (S.done) return;
+  S.r0rn[0] = res[r];
+  S.r0rn[1] = res[N + r];
 }
 
 // mixed with restart (FP16 storage): after a reliable update the recurrence
@@ -723,7 +736,13 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
   Field r1, r2, s, t, p, p2, xl, t64, auxl, aux64, tmpl, tmp64;
   Field *lo[7] = {&r1, &r2, &s, &t, &p, &p2, &xl};
   const int64_t V = x.V;
-  for (int k = 0; k < 6 && !rc; k++) rc = field_alloc(c, *lo[k], N, prec, V);
+  // The shadow residual r2 (r^ of BiCGStab) stays fixed through an FP32 solve
+  // (no restarts), and any fixed shadow vector serves when every <r2, .> uses
+  // the same one: it is kept in FP16 with a per-site scale, which halves the
+  // bytes of the two <r2, .> products per iteration.
+  using IOR = typename std::conditional<std::is_same<IOL, IOPlain<float>>::value, IOHalf, IOL>::type;
+  const int prec_r2 = std::is_same<IOR, IOHalf>::value ? CVCHIP_PREC_F16 : prec;
+  for (int k = 0; k < 6 && !rc; k++) rc = field_alloc(c, *lo[k], N, k == 1 ? prec_r2 : prec, V);
   // the solution accumulator since the last reliable update stays FP32
   if (!rc) rc = field_alloc(c, xl, N, CVCHIP_PREC_F32, V);
   if (!rc) rc = field_alloc(c, t64, N, CVCHIP_PREC_F64, V);
@@ -750,8 +769,8 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
   }
   using IO = IOL;
   using I64 = IOPlain<double>;
-  IO ir1 = mkio<IO>(r1), ir2 = mkio<IO>(r2), is = mkio<IO>(s), iT = mkio<IO>(t), ip = mkio<IO>(p),
-     ip2 = mkio<IO>(p2);
+  IO ir1 = mkio<IO>(r1), is = mkio<IO>(s), iT = mkio<IO>(t), ip = mkio<IO>(p), ip2 = mkio<IO>(p2);
+  IOR ir2 = mkio<IOR>(r2);
   IOPlain<float> ixl = mkio<IOPlain<float>>(xl);
   I64 ix = mkio<I64>(x), ib = mkio<I64>(b), it64 = mkio<I64>(t64);
   const double delta2 = delta * delta;
@@ -769,10 +788,17 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
   // r = b - D x in FP64, converted; r2 = p = r; xl = 0
   TRY(op64.apply(t64, x, DOT_NONE, nullptr, nullptr));
   dcalls++;
-  hipLaunchKernelGGL((k_residual<I64, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib, it64, 1,
+  hipLaunchKernelGGL((k_residual<I64, IO, IOR>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib, it64, 1,
                      S.st, 0, S.VN, N, (double *)c->red.ptr);
   TRY(reduce(S, (int)S.nblk, 2));
   hipLaunchKernelGGL(k_sc_init, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, sp, 1);
+  if constexpr (!std::is_same<IOR, IO>::value) {
+    // r0rn = <r2, r> with the stored (FP16) shadow residual, not |r|^2
+    hipLaunchKernelGGL((k_dot3<IOR, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir2, ir1, S.VN, N,
+                       (double *)c->red.ptr);
+    TRY(reduce(S, (int)S.nblk, 3));
+    hipLaunchKernelGGL(k_sc_set_r0rn, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
+  }
   CVC_HIP(hipMemsetAsync(xl.data, 0, xl.bytes, c->stream));
   OpMask mask(c, S.st);
   TRY(pull_state(S));
@@ -788,7 +814,7 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
       TRY(opl.apply_ts(t, s, c, N, S.res));
       dcalls++;
       hipLaunchKernelGGL(k_sc_omega, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
-      hipLaunchKernelGGL((k_r1_update<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, is, iT, ir2, S.st,
+      hipLaunchKernelGGL((k_r1_update<IO, IOR>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, is, iT, ir2, S.st,
                          S.VN, N, (double *)c->red.ptr, S.nt);
       TRY(reduce(S, (int)S.nblk, 3));
       hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it, 0, 1, delta2);
@@ -817,7 +843,7 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
                            S.VN, N);
         TRY(op64.apply(t64, x, DOT_NONE, nullptr, nullptr));
         dcalls++;
-        hipLaunchKernelGGL((k_residual<I64, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib,
+        hipLaunchKernelGGL((k_residual<I64, IO, IOR>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir1, ir2, ip, ib,
                            it64, restart ? 1 : 0, S.st, 1, S.VN, N, (double *)c->red.ptr);
         TRY(reduce(S, (int)S.nblk, 2));
         hipLaunchKernelGGL(k_sc_reliable, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N, it);
@@ -835,7 +861,7 @@ static int bicgstab_mixed(cvchip_ctx *c, Field &x, const Field &b, double sp, in
           hipLaunchKernelGGL(k_sc_restart, dim3(1), dim3(256), 0, c->stream, S.st, N);
         } else {
           // r0rn and beta with the replaced residual: <r2, r>
-          hipLaunchKernelGGL((k_dot3<IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir2, ir1, S.VN, N,
+          hipLaunchKernelGGL((k_dot3<IOR, IO>), dim3(S.nblk), dim3(S.B), 0, c->stream, ir2, ir1, S.VN, N,
                              (double *)c->red.ptr);
           TRY(reduce(S, (int)S.nblk, 3));
           hipLaunchKernelGGL(k_sc_r0rn, dim3(1), dim3(256), 0, c->stream, S.st, S.res, N);
