@@ -58,14 +58,9 @@ static size_t elem_bytes(int prec) {
   return prec == CVCHIP_PREC_F64 ? sizeof(double2) : prec == CVCHIP_PREC_F32 ? sizeof(float2) : sizeof(__half2);
 }
 
-int pool_get(cvchip_ctx *c, void **p, size_t bytes) {
-  for (size_t k = c->idle.size(); k-- > 0;)
-    if (c->idle[k].second == bytes) {   // the most recently freed buffer of this size
-      *p = c->idle[k].first;
-      c->idle_bytes -= bytes;
-      c->idle.erase(c->idle.begin() + (std::ptrdiff_t)k);
-      return 0;
-    }
+// hipMalloc that, when the device is full, gives the context's idle buffers
+// back and tries once more
+static int device_malloc(cvchip_ctx *c, void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess && !c->idle.empty()) {
     (void)hipGetLastError();   // clear the failed allocation's error, then retry without idle buffers
@@ -79,6 +74,17 @@ int pool_get(cvchip_ctx *c, void **p, size_t bytes) {
     return CVCHIP_ERR_OOM;
   }
   return 0;
+}
+
+int pool_get(cvchip_ctx *c, void **p, size_t bytes) {
+  for (size_t k = c->idle.size(); k-- > 0;)
+    if (c->idle[k].second == bytes) {   // the most recently freed buffer of this size
+      *p = c->idle[k].first;
+      c->idle_bytes -= bytes;
+      c->idle.erase(c->idle.begin() + (std::ptrdiff_t)k);
+      return 0;
+    }
+  return device_malloc(c, p, bytes);
 }
 
 // Buffers return to the pool in stream order: every use of a context's
@@ -120,8 +126,15 @@ int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites, bool
   f.bytes = (size_t)12 * f.V * nrhs * elem_bytes(prec);
   if (!pooled) {
     f.pool = nullptr;
-    CVC_HIP(hipMalloc(&f.data, f.bytes));
-    if (prec == CVCHIP_PREC_F16) CVC_HIP(hipMalloc(&f.norm, sizeof(float) * f.V * nrhs));
+    CVC_TRY(device_malloc(c, &f.data, f.bytes));
+    if (prec == CVCHIP_PREC_F16) {
+      const int rc = device_malloc(c, (void **)&f.norm, sizeof(float) * f.V * nrhs);
+      if (rc) {
+        (void)hipFree(f.data);
+        f.data = nullptr;
+        return rc;
+      }
+    }
     return 0;
   }
   f.pool = c;
