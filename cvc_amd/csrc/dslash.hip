@@ -948,12 +948,13 @@ static bool march_geometry(const cvchip_ctx *c, int N, Traversal &t, bool explic
 // and 20-23 the brick's t and x extents of mode 4 (0: 4); for mode 5, bits
 // 16-19 are log2 of the column's z extent (0: 8) and bits 20-23 its x extent
 // (0: 4).
-// lowp: the FP32 operator, whose automatic traversal is the t march with
+// K: complexes per 16 bytes of the operator's spinors (1 FP64, 2 FP32, 4
+// FP16+norm).  The FP32 operator's automatic traversal is the t march with
 // marches of 4 slices for 8 <= N <= 20 (at 64x32^3, per 12 rhs: 1.72-1.80 ms
 // against 1.92-2.06 ms for the best bricks with 12 rhs, 1.74 against 1.91 ms
 // with 16) and bricks beyond (24 rhs: 1.83 ms against 1.93 ms for the march);
 // the FP64 and FP16 operators keep the bricks
-static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false, bool lowp = false) {
+static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false, int K = 1) {
   Lattice L = c->lat;
   if (eo) {   // the half lattice of one parity: LZ/2 half-sites per z line
     L.LZ /= 2;
@@ -967,7 +968,7 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false, bool low
     if (!eo && march_geometry(c, N, t, true, 16)) return t;
     want = 4;   // the even/odd operator does not march
   }
-  if (want == 0 && lowp && !eo && N >= 8 && N <= 20) {
+  if (want == 0 && K == 2 && !eo && N >= 8 && N <= 20) {
     Traversal t{};
     if (march_geometry(c, N, t, false, 4)) return t;
   }
@@ -977,17 +978,17 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false, bool low
   // with 12 rhs: 3.98 and 4.19 ms against 4.75 ms for XCD chunks of a
   // slice); XCD chunks of a slice for a single rhs (0.48 ms against 0.54).
   // The brick's y extent shrinks as N grows so that the spinor bytes of a
-  // brick's x-y cross-section stay about constant: yb * N <= 96 in FP64
-  // (measured at 64x32^3: 12 rhs best at yb 8, 24 rhs at 4, 48 rhs at 2) and
-  // yb * N <= 192 for the half-size FP32 complexes (24 rhs best at yb 8,
-  // 48 rhs at 4: 1.86 ms per 12 rhs against 1.96-1.99 ms at yb 2).
+  // brick's x-y cross-section stay about constant, yb * N <= 96 K: measured at
+  // 64x32^3, FP64 12 rhs best at yb 8, 24 rhs at 4, 48 rhs at 2; FP32 24 rhs
+  // at 8, 48 rhs at 4 (1.86 ms per 12 rhs against 1.96-1.99 ms at yb 2); FP16
+  // 12 rhs at 16 (1.18 ms against 1.20 ms at yb 8).
   const bool autom = want == 0;
   if (autom) want = N >= 8 ? 4 : 1;
   if (yb == 0) {
     yb = 1;
     if (autom && want == 4) {
-      const uint32_t budget = lowp ? 192u : 96u;
-      for (uint32_t y : {8u, 4u, 2u})
+      const uint32_t budget = 96u * (uint32_t)K;
+      for (uint32_t y : {16u, 8u, 4u, 2u})
         if (y * (uint32_t)N <= budget && L.LY % (int)y == 0) { yb = y; break; }
     } else {
       for (uint32_t y : {16u, 8u, 4u, 2u})
@@ -1060,9 +1061,12 @@ static Traversal traversal(const cvchip_ctx *c, int N, bool eo = false, bool low
 }
 
 unsigned dslash_blocks(const cvchip_ctx *c, int N) {
-  const unsigned a = std::max(traversal(c, N).nblk, traversal(c, N, false, true).nblk);
-  const unsigned b = c->lat.LZ % 2 == 0 ? traversal(c, N, true).nblk : 0u;
-  return a > b ? a : b;
+  unsigned m = 0;
+  for (int K : {1, 2, 4}) {
+    m = std::max(m, traversal(c, N, false, K).nblk);
+    if (c->lat.LZ % 2 == 0) m = std::max(m, traversal(c, N, true, K).nblk);
+  }
+  return m;
 }
 
 struct EOArgs {
@@ -1096,7 +1100,7 @@ static int launch(cvchip_ctx *c, Field &out, const Field &in, double mutm, int g
     P.invS = 1.0f / (float)S;
     P.invKS = 1.0f / (float)(KLs * S);
   }
-  const Traversal tr = traversal(c, N, EO, IOI::K == 2);
+  const Traversal tr = traversal(c, N, EO, IOI::K);
   P.csz = tr.csz;
   P.bpc = tr.bpc;
   P.mode = tr.mode;

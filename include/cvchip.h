@@ -92,10 +92,11 @@ int cvchip_ctx_set_solver_flags(cvchip_ctx *ctx, unsigned flags);
  * 5); bits 16-19 bt of traversal 4 (0: 4) or log2 zb of traversal 5 (0: zb 8)
  * and 20-23 bx of traversals 4 and 5 (0: 4); bits 24-27 extra LDS per block in
  * 8 KiB units (caps the resident blocks).  Default 5: links in LDS,
- * streaming stores, automatic traversal (bricks 4x4x8 for 8 <= N <= 12,
- * 4x4x4 for N up to 24, 4x4x2 beyond; for the FP32 operator the t march over
- * 4 slices in patches of 8x8 columns for 8 <= N <= 20, and bricks twice as
- * high as the FP64 ones beyond).  The operator output
+ * streaming stores, automatic traversal (for N >= 8 bricks of 4 slices x 4
+ * x-planes x yb y-rows, yb the largest of 16, 8, 4, 2 dividing LY with
+ * yb * N <= 96 K, K = 1, 2, 4 for FP64, FP32, FP16+norm spinors; for the FP32
+ * operator with 8 <= N <= 20 the t march over 4 slices in patches of 8x8
+ * columns).  The operator output
  * does not depend on the traversal up to rounding (the t march sums the
  * diagonal term first); the fused dot products of the solvers differ at
  * rounding level. */

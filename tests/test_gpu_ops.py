@@ -127,8 +127,57 @@ def test_dslash_policies(chip, orc, nrhs):
         ctx.set_dslash_policy(0x350141)     # brick t extent 5 does not divide T = 16
 
 
+@pytest.mark.parametrize("nrhs", [12, 24])
+def test_dslash_tall_bricks(chip, orc, nrhs):
+    """A lattice on which the bricks tile at every automatic height (LX/4 *
+    LY/yb a multiple of the 8 XCDs for yb = 16, 8, 4, 2): the automatic
+    traversal picks yb * N <= 96 K for spinors of K complexes per 16 bytes
+    (FP64 yb 8, FP32 8 at 24 rhs and the t march at 12, FP16 16 at 12 rhs),
+    and explicit bricks 16 rows high give the oracle's operator too; the FP64
+    bits do not depend on the brick shape."""
+    dims = (4, 32, 16, 4)
+    print(f"dslash tall bricks nrhs={nrhs}", flush=True)
+    g = chip.gauge_random(dims, 8, 0.3)
+    ctx = chip.Context(dims, 0.1632, 0.004, BC, gauge=g)
+    L = orc.Lattice(dims, 0.1632, 0.004, BC)
+    rng = np.random.default_rng(40 + nrhs)
+    phi = rand_spinor(rng, L.V, nrhs)
+    want = [L.Q_phi(g, phi[r]) for r in range(nrhs)]
+    fin, fout = ctx.field(nrhs), ctx.field(nrhs)
+    fin.upload(phi)
+    first = None
+    for pol in (0x5, 0x1045, 0x21045, 0x221045, 0x845):   # yb 16 with bt 4 / 2 and bx 4 / 2; yb 8
+        ctx.set_dslash_policy(pol)
+        ctx.dslash(fout, fin)
+        got = fout.download()
+        for r in range(nrhs):
+            assert rel_err(got[r], want[r]) < 1e-13, (hex(pol), r)
+        if first is None:
+            first = got
+        else:
+            assert np.array_equal(got, first), hex(pol)
+        for prec, tol in [(chip.PREC_F32, 2e-6), (chip.PREC_F16, 2e-3)]:
+            a, b, back = ctx.field(nrhs, prec), ctx.field(nrhs, prec), ctx.field(nrhs)
+            a.copy_from(fin)
+            ctx.dslash(b, a)
+            back.copy_from(b)
+            lo = back.download()
+            for r in range(nrhs):
+                assert rel_err(lo[r], want[r]) < tol, (hex(pol), prec, r)
+    ctx.set_dslash_policy(0x5)
+    # even/odd solves on the half lattice (whose bricks take the same heights)
+    b = rand_spinor(np.random.default_rng(7), L.V, 12)
+    for prec, delta in [(chip.PREC_F32, 0.01), (chip.PREC_F16, 0.02)]:
+        rc, x, st = ctx.invert_Qtm_her(b, 1e-20, 5000, prec=prec, delta=delta, eo=True)
+        print(f"  even/odd prec {prec}: {rc} iterations", flush=True)
+        assert st["converged"] == 12
+        for a in (0, 11):
+            res = L.Q_phi(g, x[a]) - b[a]
+            assert L.dot_re(res, res) / L.dot_re(b[a], b[a]) < 1e-19
+
+
 # t march (traversal 6): march length bits 8-15, patch x / y extents bits 16-19 / 20-23
-MARCH = [0x65, 0x0165, 0x0465, 0x110865, 0x221065, 0x420365, 0x64]
+MARCH =[0x65, 0x0165, 0x0465, 0x110865, 0x221065, 0x420365, 0x64]
 
 
 @pytest.mark.parametrize("dims,nrhs", [((16, 4, 8, 6), 12), ((8, 8, 8, 8), 12), ((6, 4, 4, 8), 5),
