@@ -172,9 +172,9 @@ int ensure_red2(cvchip_ctx *c, size_t bytes);
 // pooled = false, so they own their memory outright.
 int field_alloc(cvchip_ctx *c, Field &f, int nrhs, int prec, int64_t sites = 0, bool pooled = true);
 void field_free(Field &f);
-// device memory from the context's idle pool (or hipMalloc; on failure the
-// pool is released and the allocation retried, also for the sets that own
-// their memory); back into the pool
+// device memory from the context's idle pool (or hipMalloc; on failure idle
+// buffers are released, oldest first, until the allocation fits, also for the
+// sets that own their memory); back into the pool
 constexpr size_t kPoolIdleMax = (size_t)160 << 30;   // of 288 GB: the workspaces of a 48-rhs solve stay
 int pool_get(cvchip_ctx *c, void **p, size_t bytes);
 void pool_put(cvchip_ctx *c, void *p, size_t bytes);
