@@ -63,34 +63,55 @@ void die(int rc, const char *where) {
   exit(1);
 }
 
-// Hash of every 64-bit word of the gauge field (four independent
-// multiply-xorshift lanes, folded at the end).  Every step is a bijection of a
-// lane's state, so a change of one word (one link entry, also a single bit)
-// always changes the hash; changes of several words are missed only by a
-// chance collision of the 64-bit value.  In place edits of
-// g_gauge_field (a local gauge transformation as in apply_gt_gauge,
-// cvc_utils.c:2044-2068, or a one-link finite difference) are seen by the next
-// operator or solver call.
-uint64_t gauge_hash(const double *g, long n) {
-  const uint64_t K = 0x9e3779b97f4a7c15ULL;
-  uint64_t h[4] = {1, 2, 3, 4};
+const uint64_t kHashMul = 0x9e3779b97f4a7c15ULL;
+
+// one block of words: four independent multiply-xorshift lanes, folded
+uint64_t block_hash(const double *g, long n, uint64_t seed) {
+  uint64_t h[4] = {seed + 1, seed + 2, seed + 3, seed + 4};
   long i = 0;
   for (; i + 4 <= n; i += 4) {
     for (int l = 0; l < 4; l++) {
       uint64_t w;
       memcpy(&w, g + i + l, sizeof(w));
-      h[l] = (h[l] ^ w) * K;
+      h[l] = (h[l] ^ w) * kHashMul;
       h[l] ^= h[l] >> 29;
     }
   }
   for (; i < n; i++) {
     uint64_t w;
     memcpy(&w, g + i, sizeof(w));
-    h[0] = (h[0] ^ w) * K;
+    h[0] = (h[0] ^ w) * kHashMul;
     h[0] ^= h[0] >> 29;
   }
   uint64_t r = (uint64_t)n;
-  for (int l = 0; l < 4; l++) r = (r ^ h[l]) * K;
+  for (int l = 0; l < 4; l++) r = (r ^ h[l]) * kHashMul;
+  return r ^ (r >> 31);
+}
+
+// Hash of every 64-bit word of the gauge field: 256 fixed blocks hashed by
+// the program's OpenMP threads (the blocks, and so the value, do not depend on
+// the number of threads), the block values folded in block order.  Every step
+// is a bijection of the state in the word or block value it takes in, so a
+// change of one word (one link entry, also a single bit) always changes the
+// hash; changes of several words are missed only by a chance collision of the
+// 64-bit value.  In place edits of g_gauge_field (a local gauge transformation
+// as in apply_gt_gauge, cvc_utils.c:2044-2068, or a one-link finite
+// difference) are seen by the next operator or solver call.
+uint64_t gauge_hash(const double *g, long n) {
+  constexpr int kBlocks = 256;
+  uint64_t hb[kBlocks];
+  const long per = (n + kBlocks - 1) / kBlocks;
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < kBlocks; b++) {
+    const long lo = (long)b * per < n ? (long)b * per : n;
+    const long hi = lo + per < n ? lo + per : n;
+    hb[b] = block_hash(g + lo, hi - lo, (uint64_t)b << 8);
+  }
+  uint64_t r = (uint64_t)n;
+  for (int b = 0; b < kBlocks; b++) {
+    r = (r ^ hb[b]) * kHashMul;
+    r ^= r >> 29;
+  }
   return r ^ (r >> 31);
 }
 
