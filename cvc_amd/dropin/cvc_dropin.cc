@@ -26,6 +26,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <thread>
+#include <vector>
+
 #include "cvc_dropin.h"
 #include "cvchip.h"
 
@@ -88,25 +91,33 @@ uint64_t block_hash(const double *g, long n, uint64_t seed) {
   return r ^ (r >> 31);
 }
 
-// Hash of every 64-bit word of the gauge field: 256 fixed blocks hashed by
-// the program's OpenMP threads (the blocks, and so the value, do not depend on
-// the number of threads), the block values folded in block order.  Every step
-// is a bijection of the state in the word or block value it takes in, so a
-// change of one word (one link entry, also a single bit) always changes the
-// hash; changes of several words are missed only by a chance collision of the
-// 64-bit value.  In place edits of g_gauge_field (a local gauge transformation
-// as in apply_gt_gauge, cvc_utils.c:2044-2068, or a one-link finite
-// difference) are seen by the next operator or solver call.
+// Hash of every 64-bit word of the gauge field: 256 fixed blocks hashed by up
+// to 16 host threads (the blocks, and so the value, do not depend on the
+// number of threads), the block values folded in block order.  Every step is a
+// bijection of the state in the word or block value it takes in, so a change
+// of one word (one link entry, also a single bit) always changes the hash;
+// changes of several words are missed only by a chance collision of the
+// 64-bit value.  In place edits of g_gauge_field (a local gauge
+// transformation as in apply_gt_gauge, cvc_utils.c:2044-2068, or a one-link
+// finite difference) are seen by the next operator or solver call.
 uint64_t gauge_hash(const double *g, long n) {
   constexpr int kBlocks = 256;
   uint64_t hb[kBlocks];
   const long per = (n + kBlocks - 1) / kBlocks;
-#pragma omp parallel for schedule(static)
-  for (int b = 0; b < kBlocks; b++) {
-    const long lo = (long)b * per < n ? (long)b * per : n;
-    const long hi = lo + per < n ? lo + per : n;
-    hb[b] = block_hash(g + lo, hi - lo, (uint64_t)b << 8);
-  }
+  auto blocks = [&](unsigned first, unsigned step) {
+    for (unsigned b = first; b < (unsigned)kBlocks; b += step) {
+      const long lo = (long)b * per < n ? (long)b * per : n;
+      const long hi = lo + per < n ? lo + per : n;
+      hb[b] = block_hash(g + lo, hi - lo, (uint64_t)b << 8);
+    }
+  };
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt < 1 || n < (1L << 20)) nt = 1;
+  if (nt > 16) nt = 16;
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < nt; t++) pool.emplace_back(blocks, t, nt);
+  blocks(0, nt);
+  for (auto &th : pool) th.join();
   uint64_t r = (uint64_t)n;
   for (int b = 0; b < kBlocks; b++) {
     r = (r ^ hb[b]) * kHashMul;
