@@ -346,9 +346,14 @@ class Context:
         _check(lib().cvchip_set_gauge(self._h, _ptr(g)))
 
     # -- host-array drop-ins ---------------------------------------------------
-    def Q_phi(self, phi, mutm=None, gamma5_out=False):
+    def Q_phi(self, phi, mutm=None, gamma5_out=False, out=None):
+        """xi = D(mutm) phi on host arrays [V][24] (Q_phi_tbc / Q_phi / Qf5); out, if given, is a
+        C-contiguous float64 [V][24] array written in place (as the reference's xi)."""
         phi = np.ascontiguousarray(phi, dtype=np.float64)
-        out = np.empty((self.V, 24))
+        if out is None:
+            out = np.empty((self.V, 24))
+        elif out.dtype != np.float64 or not out.flags.c_contiguous or out.size != 24 * self.V:
+            raise ValueError("out must be a C-contiguous float64 array of V x 24 entries")
         _check(lib().cvchip_Q_phi(self._h, _ptr(out), _ptr(phi), self.mu if mutm is None else float(mutm),
                                   int(bool(gamma5_out))))
         return out

@@ -315,10 +315,12 @@ int contract_fields(cvchip_ctx *c, double *conn, double contact[8], const Field 
     hipLaunchKernelGGL(k_contact, dim3(1), dim3(64), 0, c->stream, dct, (const double2 *)up.data, hL, y,
                        (const uint32_t *)dysp, V);
     double2 hct[96];
-    e = hipMemcpyAsync(conn, dconn, sizeof(double2) * 16 * (size_t)V, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hct, dct, sizeof(hct), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) rc = fail(e, "contraction results");
+    rc = copy_to_host(c, conn, dconn, sizeof(double2) * 16 * (size_t)V);
+    if (!rc) {
+      e = hipMemcpyAsync(hct, dct, sizeof(hct), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) rc = fail(e, "contraction results");
+    }
     if (!rc) {
       // accumulate in the reference's order: part 1 over (ir, ia), then part 2
       for (int nu = 0; nu < 4; nu++) {

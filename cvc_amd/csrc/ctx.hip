@@ -5,6 +5,10 @@
 #include <string.h>
 #include <math.h>
 
+#include <algorithm>
+#include <thread>
+#include <vector>
+
 #include "spinor_io.h"
 
 namespace cvchip {
@@ -31,6 +35,85 @@ int ensure_stage(cvchip_ctx *c, size_t bytes) {
   c->stage_bytes = 0;
   CVC_HIP(hipMalloc(&c->stage, bytes));
   c->stage_bytes = bytes;
+  return 0;
+}
+
+// Device-to-host copies of 16 MB and more go through two pinned host buffers
+// in turn: the copy of chunk i out of its pinned buffer (over up to 8 host
+// threads) overlaps the DMA of chunk i + 1.  A direct copy into pageable memory
+// moves 403 MB in about 23 ms (17 GB/s).
+constexpr size_t kPinChunk = (size_t)64 << 20;
+constexpr size_t kPinMin = (size_t)16 << 20;
+
+static void par_memcpy(void *dst, const void *src, size_t bytes) {
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = std::max(1u, std::min(8u, nt));
+  nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, bytes / ((size_t)4 << 20)));
+  const size_t per = (bytes + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; t++) {
+    const size_t lo = (size_t)t * per;
+    if (lo >= bytes) break;
+    const size_t len = std::min(per, bytes - lo);
+    th.emplace_back([=] { memcpy((char *)dst + lo, (const char *)src + lo, len); });
+  }
+  memcpy(dst, src, std::min(per, bytes));
+  for (auto &x : th) x.join();
+}
+
+static int ensure_pinned(cvchip_ctx *c) {
+  for (int k = 0; k < 2; k++) {
+    if (!c->pin[k]) CVC_HIP(hipHostMalloc(&c->pin[k], kPinChunk, hipHostMallocDefault));
+    if (!c->pin_ev[k]) CVC_HIP(hipEventCreateWithFlags(&c->pin_ev[k], hipEventDisableTiming));
+  }
+  return 0;
+}
+
+int copy_to_host(cvchip_ctx *c, void *host, const void *dev, size_t bytes) {
+  if (bytes < kPinMin) {
+    CVC_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  // a DMA into a pinned buffer follows, in stream order, the one that last read it
+  CVC_TRY(ensure_pinned(c));
+  const size_t n = (bytes + kPinChunk - 1) / kPinChunk;
+  auto issue = [&](size_t i) -> int {
+    const size_t off = i * kPinChunk, len = std::min(kPinChunk, bytes - off);
+    CVC_HIP(hipMemcpyAsync(c->pin[i % 2], (const char *)dev + off, len, hipMemcpyDeviceToHost, c->stream));
+    CVC_HIP(hipEventRecord(c->pin_ev[i % 2], c->stream));
+    return 0;
+  };
+  for (size_t i = 0; i < n && i < 2; i++) CVC_TRY(issue(i));
+  for (size_t i = 0; i < n; i++) {
+    CVC_HIP(hipEventSynchronize(c->pin_ev[i % 2]));
+    const size_t off = i * kPinChunk, len = std::min(kPinChunk, bytes - off);
+    par_memcpy((char *)host + off, c->pin[i % 2], len);
+    if (i + 2 < n) CVC_TRY(issue(i + 2));   // this buffer is free again
+  }
+  return 0;
+}
+
+// Host-to-device copies of 16 MB and more: each chunk is copied into a pinned
+// buffer (up to 8 host threads) while the DMA of the previous chunk runs (a
+// direct copy from pageable memory moves 403 MB in about 13.5 ms, 30 GB/s).
+// The host buffer has been read when the call returns; the DMAs are enqueued on
+// the context's stream.
+int copy_to_device(cvchip_ctx *c, void *dev, const void *host, size_t bytes) {
+  if (bytes < kPinMin) {
+    CVC_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+    return 0;
+  }
+  CVC_TRY(ensure_pinned(c));
+  const size_t n = (bytes + kPinChunk - 1) / kPinChunk;
+  for (size_t i = 0; i < n; i++) {
+    const int k = (int)(i % 2);
+    const size_t off = i * kPinChunk, len = std::min(kPinChunk, bytes - off);
+    CVC_HIP(hipEventSynchronize(c->pin_ev[k]));   // the last DMA from or into this buffer has ended
+    par_memcpy(c->pin[k], (const char *)host + off, len);
+    CVC_HIP(hipMemcpyAsync((char *)dev + off, c->pin[k], len, hipMemcpyHostToDevice, c->stream));
+    CVC_HIP(hipEventRecord(c->pin_ev[k], c->stream));
+  }
   return 0;
 }
 
@@ -424,6 +507,10 @@ extern "C" int cvchip_ctx_destroy(cvchip_ctx *c) {
   if (c->gauge64r12) (void)hipFree(c->gauge64r12);
   if (c->gauge32r12) (void)hipFree(c->gauge32r12);
   if (c->stage) (void)hipFree(c->stage);
+  for (int k = 0; k < 2; k++) {
+    if (c->pin[k]) (void)hipHostFree(c->pin[k]);
+    if (c->pin_ev[k]) (void)hipEventDestroy(c->pin_ev[k]);
+  }
   if (c->red.ptr) (void)hipFree(c->red.ptr);
   if (c->red2.ptr) (void)hipFree(c->red2.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -515,7 +602,7 @@ extern "C" int cvchip_set_gauge(cvchip_ctx *c, const double *gauge) {
   if (!c->gauge64r12) CVC_HIP(hipMalloc(&c->gauge64r12, sizeof(double2) * 24 * V));
   if (!c->gauge32r12) CVC_HIP(hipMalloc(&c->gauge32r12, sizeof(float2) * 24 * V));
   CVC_TRY(ensure_stage(c, sizeof(double) * 72 * V));
-  CVC_HIP(hipMemcpyAsync(c->stage, gauge, sizeof(double) * 72 * V, hipMemcpyHostToDevice, c->stream));
+  CVC_TRY(copy_to_device(c, c->stage, gauge, sizeof(double) * 72 * V));
   const int64_t n = 36 * V;
   hipLaunchKernelGGL(k_gauge_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                      c->gauge64, c->gauge64r12, c->gauge32r12, (const double *)c->stage, V);
@@ -536,7 +623,7 @@ int upload_aos(cvchip_ctx *c, Field &f, const double *host, int rhs0, int n) {
   const int64_t V = f.V;
   const size_t bytes = sizeof(double) * 24 * V * n;
   CVC_TRY(ensure_stage(c, bytes));
-  CVC_HIP(hipMemcpyAsync(c->stage, host, bytes, hipMemcpyHostToDevice, c->stream));
+  CVC_TRY(copy_to_device(c, c->stage, host, bytes));
   const int64_t tot = (int64_t)n * V * 12;
   dim3 g((unsigned)((tot + 255) / 256)), b(256);
   if (f.prec == CVCHIP_PREC_F64)
@@ -576,9 +663,7 @@ int download_aos(cvchip_ctx *c, const Field &f, double *host, int rhs0, int n) {
     return CVCHIP_ERR_ARG;
   }
   CVC_HIP(hipGetLastError());
-  CVC_HIP(hipMemcpyAsync(host, c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return copy_to_host(c, host, c->stage, bytes);
 }
 
 }  // namespace cvchip

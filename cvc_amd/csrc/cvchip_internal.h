@@ -146,6 +146,9 @@ struct cvchip_ctx {
   // staging for host <-> device transfers (AoS reference layout)
   double *stage = nullptr;
   size_t stage_bytes = 0;
+  // pinned host buffers (used in turn) and their events for device-to-host copies
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   // reduction scratch
   cvchip::Scratch red;
   cvchip::Scratch red2;   // first-stage sums of the two-stage partial reduction
@@ -159,6 +162,12 @@ struct cvchip_ctx {
 namespace cvchip {
 
 int ensure_stage(cvchip_ctx *c, size_t bytes);
+// device -> host copy on the context's stream (after the work enqueued before
+// it), complete on return; large copies stream through pinned buffers
+int copy_to_host(cvchip_ctx *c, void *host, const void *dev, size_t bytes);
+// host -> device copy enqueued on the context's stream; the host buffer has
+// been read when it returns
+int copy_to_device(cvchip_ctx *c, void *dev, const void *host, size_t bytes);
 // device address of the allocation-free two-point entry point's scratch on
 // `device` (the current device), taken once per device
 int twopoint_scratch_init(int device);
