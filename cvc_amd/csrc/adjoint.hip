@@ -640,8 +640,7 @@ extern "C" int cvchip_cvc_gradient(cvchip_ctx *c, const int src[4], double qmax,
                        (const double2 *)c->gauge64, A);
     hipok(hipGetLastError(), "force");
     std::vector<double2> hK((size_t)36 * V);
-    hipok(hipMemcpyAsync(hK.data(), K, sizeof(double2) * 36 * (size_t)V, hipMemcpyDeviceToHost, c->stream), "force");
-    hipok(hipStreamSynchronize(c->stream), "force");
+    if (!rc) rc = copy_to_host(c, hK.data(), K, sizeof(double2) * 36 * (size_t)V);
     if (!rc) {
       for (int nu = 0; nu < 4; nu++)
         for (int q = 0; q < 9; q++) {

@@ -539,18 +539,19 @@ extern "C" int cvchip_wi(cvchip_ctx *c, double *wi, const double *conn) {
   double2 *dc = nullptr, *dw = nullptr;
   CVC_HIP(hipMalloc(&dc, sizeof(double2) * 16 * V));
   hipError_t e = hipMalloc(&dw, sizeof(double2) * 4 * V);
-  if (e == hipSuccess) e = hipMemcpyAsync(dc, conn, sizeof(double2) * 16 * V, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) {
+  int rc = e == hipSuccess ? 0 : hip_fail(e, "Ward identity buffers", __FILE__, __LINE__);
+  if (!rc) rc = copy_to_device(c, dc, conn, sizeof(double2) * 16 * V);
+  if (!rc) {
     hipLaunchKernelGGL(k_wi, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c->stream, dw, (const double2 *)dc,
                        conn_params(c));
     e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "Ward identity", __FILE__, __LINE__);
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(wi, dw, sizeof(double2) * 4 * V, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (!rc) rc = copy_to_host(c, wi, dw, sizeof(double2) * 4 * V);
+  (void)hipStreamSynchronize(c->stream);   // nothing in flight before the buffers go
   (void)hipFree(dc);
   if (dw) (void)hipFree(dw);
-  if (e != hipSuccess) return hip_fail(e, "Ward identity", __FILE__, __LINE__);
-  return CVCHIP_OK;
+  return rc;
 }
 
 extern "C" int cvchip_pi_q(cvchip_ctx *c, double *piq, const double *conn, const double contact[8],
@@ -563,13 +564,11 @@ extern "C" int cvchip_pi_q(cvchip_ctx *c, double *piq, const double *conn, const
   double2 *a = nullptr, *t = nullptr;
   CVC_HIP(hipMalloc(&a, sizeof(double2) * 16 * V));
   hipError_t e = hipMalloc(&t, sizeof(double2) * 16 * V);
-  if (e == hipSuccess) e = hipMemcpyAsync(a, conn, sizeof(double2) * 16 * V, hipMemcpyHostToDevice, c->stream);
-  int rc = e == hipSuccess ? 0 : hip_fail(e, "Pi(q) upload", __FILE__, __LINE__);
+  int rc = e == hipSuccess ? 0 : hip_fail(e, "Pi(q) buffers", __FILE__, __LINE__);
+  if (!rc) rc = copy_to_device(c, a, conn, sizeof(double2) * 16 * V);
   if (!rc) rc = pi_q_device(c, a, t, contact, src);
-  if (!rc) {
-    e = hipMemcpy(piq, a, sizeof(double2) * 16 * V, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = hip_fail(e, "Pi(q) download", __FILE__, __LINE__);
-  }
+  if (!rc) rc = copy_to_host(c, piq, a, sizeof(double2) * 16 * V);
+  (void)hipStreamSynchronize(c->stream);   // nothing in flight before the buffers go
   (void)hipFree(a);
   if (t) (void)hipFree(t);
   return rc;
