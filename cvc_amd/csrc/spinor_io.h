@@ -33,6 +33,16 @@ struct IOPlain<double> {
     for (int c = 0; c < 12; c++) v[c] = p[(size_t)c * VN + idx];
   }
   __device__ __forceinline__ C load_c(int c, uint32_t idx) const { return p[(size_t)c * VN + idx]; }
+  // streaming load (no allocation in the caches): for a field read once per launch
+  __device__ __forceinline__ void load_nt(C v[12], uint32_t idx) const {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v *q = reinterpret_cast<const d2v *>(p);
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      const d2v w = __builtin_nontemporal_load(q + (size_t)c * VN + idx);
+      v[c] = make_double2(w.x, w.y);
+    }
+  }
   __device__ __forceinline__ void store(const C v[12], uint32_t idx) const {
 #pragma unroll
     for (int c = 0; c < 12; c++) p[(size_t)c * VN + idx] = v[c];
@@ -60,6 +70,16 @@ struct IOPlain<float> {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       const float4 f = q[(size_t)k * VN + idx];
+      v[2 * k] = make_float2(f.x, f.y);
+      v[2 * k + 1] = make_float2(f.z, f.w);
+    }
+  }
+  __device__ __forceinline__ void load_nt(C v[12], uint32_t idx) const {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v *q = reinterpret_cast<const f4v *>(p);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const f4v f = __builtin_nontemporal_load(q + (size_t)k * VN + idx);
       v[2 * k] = make_float2(f.x, f.y);
       v[2 * k + 1] = make_float2(f.z, f.w);
     }
@@ -95,6 +115,23 @@ struct IOHalf {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const uint4 w = q[(size_t)k * VN + idx];
+      const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        __half2 h;
+        __builtin_memcpy(&h, &u[j], 4);
+        const float2 f = __half22float2(h);
+        v[4 * k + j] = make_float2(f.x * n, f.y * n);
+      }
+    }
+  }
+  __device__ __forceinline__ void load_nt(C v[12], uint32_t idx) const {
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    const float n = __builtin_nontemporal_load(nrm + idx);
+    const u4v *q = reinterpret_cast<const u4v *>(p);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const u4v w = __builtin_nontemporal_load(q + (size_t)k * VN + idx);
       const uint32_t u[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
       for (int j = 0; j < 4; j++) {
