@@ -97,8 +97,10 @@ int copy_to_host(cvchip_ctx *c, void *host, const void *dev, size_t bytes) {
 // Host-to-device copies of 16 MB and more: each chunk is copied into a pinned
 // buffer (up to 8 host threads) while the DMA of the previous chunk runs (a
 // direct copy from pageable memory moves 403 MB in about 13.5 ms, 30 GB/s).
-// The host buffer has been read when the call returns; the DMAs are enqueued on
-// the context's stream.
+// The copies are enqueued on the context's stream; a large host buffer has
+// been read when the call returns, a small one (a plain asynchronous copy) only
+// once the stream has reached it, so callers synchronize the stream before the
+// host buffer may change.
 int copy_to_device(cvchip_ctx *c, void *dev, const void *host, size_t bytes) {
   if (bytes < kPinMin) {
     CVC_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));

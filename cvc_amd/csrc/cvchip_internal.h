@@ -165,8 +165,8 @@ int ensure_stage(cvchip_ctx *c, size_t bytes);
 // device -> host copy on the context's stream (after the work enqueued before
 // it), complete on return; large copies stream through pinned buffers
 int copy_to_host(cvchip_ctx *c, void *host, const void *dev, size_t bytes);
-// host -> device copy enqueued on the context's stream; the host buffer has
-// been read when it returns
+// host -> device copy enqueued on the context's stream; the caller
+// synchronizes the stream before the host buffer may change
 int copy_to_device(cvchip_ctx *c, void *dev, const void *host, size_t bytes);
 // device address of the allocation-free two-point entry point's scratch on
 // `device` (the current device), taken once per device
