@@ -55,7 +55,11 @@ static void par_memcpy(void *dst, const void *src, size_t bytes) {
     const size_t lo = (size_t)t * per;
     if (lo >= bytes) break;
     const size_t len = std::min(per, bytes - lo);
-    th.emplace_back([=] { memcpy((char *)dst + lo, (const char *)src + lo, len); });
+    try {
+      th.emplace_back([=] { memcpy((char *)dst + lo, (const char *)src + lo, len); });
+    } catch (...) {   // no thread to be had: this slice on the calling thread
+      memcpy((char *)dst + lo, (const char *)src + lo, len);
+    }
   }
   memcpy(dst, src, std::min(per, bytes));
   for (auto &x : th) x.join();

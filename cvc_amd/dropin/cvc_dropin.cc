@@ -115,7 +115,13 @@ uint64_t gauge_hash(const double *g, long n) {
   if (nt < 1 || n < (1L << 20)) nt = 1;
   if (nt > 16) nt = 16;
   std::vector<std::thread> pool;
-  for (unsigned t = 1; t < nt; t++) pool.emplace_back(blocks, t, nt);
+  for (unsigned t = 1; t < nt; t++) {
+    try {
+      pool.emplace_back(blocks, t, nt);
+    } catch (...) {   // no thread to be had: its blocks on the calling thread
+      blocks(t, nt);
+    }
+  }
   blocks(0, nt);
   for (auto &th : pool) th.join();
   uint64_t r = (uint64_t)n;
