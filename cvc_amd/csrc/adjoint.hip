@@ -394,290 +394,292 @@ double g5d(int a) { return a < 2 ? 1. : -1.; }
 extern "C" int cvchip_cvc_gradient(cvchip_ctx *c, const int src[4], double qmax, double sp, int niter_max, int prec,
                                    double delta, double *L_out, double *dmu_out, double *force,
                                    cvchip_stats *stats) {
-  if (!c || !src || !L_out || !dmu_out) {
-    set_error("NULL argument to cvchip_cvc_gradient");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!c->gauge_set) {
-    set_error("gauge field not set (call cvchip_set_gauge first)");
-    return CVCHIP_ERR_STATE;
-  }
-  const Lattice &Lt = c->lat;
-  const uint32_t V = (uint32_t)Lt.V;
-  const int Ls[4] = {Lt.T, Lt.LX, Lt.LY, Lt.LZ};
-  // momentum set Q: n_mu in (-L/2, L/2], |q| <= qmax
-  std::vector<double> qs;
-  for (int n0 = -(Ls[0] - 1) / 2; n0 <= Ls[0] / 2; n0++)
-    for (int n1 = -(Ls[1] - 1) / 2; n1 <= Ls[1] / 2; n1++)
-      for (int n2 = -(Ls[2] - 1) / 2; n2 <= Ls[2] / 2; n2++)
-        for (int n3 = -(Ls[3] - 1) / 2; n3 <= Ls[3] / 2; n3++) {
-          const double q[4] = {2 * M_PI * n0 / Ls[0], 2 * M_PI * n1 / Ls[1], 2 * M_PI * n2 / Ls[2],
-                               2 * M_PI * n3 / Ls[3]};
-          if (q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3] <= qmax * qmax + 1e-12)
-            for (int m = 0; m < 4; m++) qs.push_back(q[m]);
-        }
-  const int nq = (int)(qs.size() / 4);
-  AdjParams A;
-  A.P = conn_params(c);
-  A.y = site_of(c, src[0], src[1], src[2], src[3]);
-  uint32_t ysp[4];
-  for (int nu = 0; nu < 4; nu++) {
-    int d[4] = {0, 0, 0, 0};
-    d[nu] = 1;
-    ysp[nu] = site_of(c, src[0] + d[0], src[1] + d[1], src[2] + d[2], src[3] + d[3]);
-  }
-
-  Field up, dn, xu, xd, b12, x12;
-  double *cw = nullptr, *dqs = nullptr;
-  double2 *psi = nullptr, *K = nullptr, *blk = nullptr;
-  std::vector<double> conn((size_t)32 * V);
-  double contact[8];
-  cvchip_stats tot;
-  memset(&tot, 0, sizeof(tot));
-  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, xu, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, xd, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, b12, 12, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, x12, 12, CVCHIP_PREC_F64);
-  auto hipok = [&](hipError_t e, const char *w) {
-    if (e != hipSuccess && !rc) rc = hip_fail(e, w, __FILE__, __LINE__);
-  };
-  if (!rc) {
-    hipok(hipMalloc(&cw, sizeof(double) * V), "weights");
-    hipok(hipMalloc(&dqs, sizeof(double) * (qs.size() + 4)), "momenta");
-    hipok(hipMalloc(&psi, sizeof(double2) * 288 * (size_t)V), "psi");
-    hipok(hipMalloc(&K, sizeof(double2) * 36 * (size_t)V), "force");
-    hipok(hipMalloc(&blk, sizeof(double2) * 720), "site block");
-  }
-  const int nblk_dn = (int)((V + 19) / 20);
-  if (!rc) rc = ensure_red(c, sizeof(double) * ((size_t)288 * nblk_dn + 64 * 256));
-  // ---- forward: propagators, conn, L
-  if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
-  if (!rc) rc = contract_fields(c, conn.data(), contact, up, dn, src);
-  SrcLinks hL;
-  if (!rc) rc = source_links(c, src, &hL);
-  if (!rc) {
-    hipok(hipMemcpyAsync(dqs, qs.data(), sizeof(double) * qs.size(), hipMemcpyHostToDevice, c->stream), "momenta");
-    hipLaunchKernelGGL(k_weight, dim3((V + 255) / 256), dim3(256), 0, c->stream, cw, (const double *)dqs, nq, A,
-                       src[0], src[1], src[2], src[3]);
-    hipok(hipMemsetAsync(xu.data, 0, xu.bytes, c->stream), "xi");
-    hipok(hipMemsetAsync(xd.data, 0, xd.bytes, c->stream), "xi");
-    hipok(hipMemsetAsync(K, 0, sizeof(double2) * 36 * (size_t)V, c->stream), "force");
-  }
-  std::vector<double> hcw(V);
-  if (!rc) {
-    hipok(hipMemcpyAsync(hcw.data(), cw, sizeof(double) * V, hipMemcpyDeviceToHost, c->stream), "weights");
-    hipok(hipStreamSynchronize(c->stream), "weights");
-  }
-  double Lval = 0.;
-  if (!rc) {
-    for (uint32_t x = 0; x < V; x++)
-      for (int mu = 0; mu < 4; mu++) Lval += -4.0 * hcw[x] * conn[2 * ((size_t)16 * x + 5 * mu)];
-    for (int mu = 0; mu < 4; mu++) Lval -= nq * contact[2 * mu];
-  }
-  // ---- adjoint sources, explicit sink force, source bilinears (per nu = mu)
-  cd Ksrc[4][9];
-  for (int nu = 0; nu < 4; nu++)
-    for (int q = 0; q < 9; q++) Ksrc[nu][q] = 0.;
-  std::vector<double> bil(288);
-  double *dres = nullptr;
-  if (!rc) hipok(hipMalloc(&dres, sizeof(double) * 288), "bilinears");
-  for (int nu = 0; nu < 4 && !rc; nu++) {
-    rc = launch_psi(c, psi, dn, hL, nu);
-    if (rc) break;
-    hipLaunchKernelGGL(k_xi_up, dim3((V * 12u + 255) / 256), dim3(256), 0, c->stream, (double2 *)xu.data,
-                       (const double2 *)psi, (const double *)cw, (const double2 *)c->gauge64, A, nu);
-    hipLaunchKernelGGL(k_xi_dn, dim3(nblk_dn), dim3(240), 0, c->stream, (double2 *)xd.data, (double *)c->red.ptr,
-                       (const double2 *)up.data, (const double2 *)dn.data, (const double *)cw,
-                       (const double2 *)c->gauge64, hL, A, nu);
-    hipok(hipGetLastError(), "adjoint sources");
-    if (!rc) rc = reduce_partials(c, (const double *)c->red.ptr, nblk_dn, 288, 1, dres);
-    hipLaunchKernelGGL(k_force_sink, dim3((V + 255) / 256), dim3(256), 0, c->stream, K, (const double2 *)up.data,
-                       (const double2 *)psi, (const double *)cw, (const double2 *)c->gauge64, A, nu);
-    hipok(hipGetLastError(), "sink force");
-    hipok(hipMemcpyAsync(bil.data(), dres, sizeof(double) * 288, hipMemcpyDeviceToHost, c->stream), "bilinears");
-    hipok(hipStreamSynchronize(c->stream), "bilinears");
-    if (rc) break;
-    // source-link derivative through the vertices: K += Us N1 - N2 Us^+
-    const double2 *Us = hL.u[nu];
-    cd U3[9];
-    for (int q = 0; q < 9; q++) U3[q] = cd(Us[q].x, Us[q].y);
-    cd N1[9], N2[9];
-    for (int q = 0; q < 9; q++) N1[q] = N2[q] = 0.;
-    for (int l = 0; l < 12; l++) {
-      const int j = l / 3, ib = l % 3;
-      const int ira = GCOL[nu][j];
-      const cd ea = gmat(nu, ira, j) * g5d(j);
-      const double s5 = g5d(j);
-      for (int w = 0; w < 2; w++) {
-        const cd e1 = w == 0 ? ea : cd(-s5, 0), e2 = w == 0 ? ea : cd(s5, 0);
-        for (int ia = 0; ia < 3; ia++) {
-          const cd b1(bil[l * 24 + 2 * (3 * w + ia)], bil[l * 24 + 2 * (3 * w + ia) + 1]);
-          const cd b2(bil[l * 24 + 12 + 2 * (3 * w + ia)], bil[l * 24 + 12 + 2 * (3 * w + ia) + 1]);
-          N1[3 * ib + ia] += e1 * b1;
-          N2[3 * ib + ia] += e2 * b2;
-        }
-      }
+  return cvchip::guarded("cvchip_cvc_gradient", [&]() -> int {
+    if (!c || !src || !L_out || !dmu_out) {
+      set_error("NULL argument to cvchip_cvc_gradient");
+      return CVCHIP_ERR_NULL;
     }
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) {
-        cd s1 = 0., s2 = 0.;
-        for (int m = 0; m < 3; m++) {
-          s1 += U3[3 * a + m] * N1[3 * m + b];
-          s2 += N2[3 * a + m] * std::conj(U3[3 * b + m]);
-        }
-        Ksrc[nu][3 * a + b] += s1 - s2;
-      }
-  }
-  // ---- contact term: adjoint sources at y+nu (up from y) and y (up from y+nu), explicit source-link part
-  if (!rc) {
-    std::vector<double2> su4(720), suy(720);
+    if (!c->gauge_set) {
+      set_error("gauge field not set (call cvchip_set_gauge first)");
+      return CVCHIP_ERR_STATE;
+    }
+    const Lattice &Lt = c->lat;
+    const uint32_t V = (uint32_t)Lt.V;
+    const int Ls[4] = {Lt.T, Lt.LX, Lt.LY, Lt.LZ};
+    // momentum set Q: n_mu in (-L/2, L/2], |q| <= qmax
+    std::vector<double> qs;
+    for (int n0 = -(Ls[0] - 1) / 2; n0 <= Ls[0] / 2; n0++)
+      for (int n1 = -(Ls[1] - 1) / 2; n1 <= Ls[1] / 2; n1++)
+        for (int n2 = -(Ls[2] - 1) / 2; n2 <= Ls[2] / 2; n2++)
+          for (int n3 = -(Ls[3] - 1) / 2; n3 <= Ls[3] / 2; n3++) {
+            const double q[4] = {2 * M_PI * n0 / Ls[0], 2 * M_PI * n1 / Ls[1], 2 * M_PI * n2 / Ls[2],
+                                 2 * M_PI * n3 / Ls[3]};
+            if (q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3] <= qmax * qmax + 1e-12)
+              for (int m = 0; m < 4; m++) qs.push_back(q[m]);
+          }
+    const int nq = (int)(qs.size() / 4);
+    AdjParams A;
+    A.P = conn_params(c);
+    A.y = site_of(c, src[0], src[1], src[2], src[3]);
+    uint32_t ysp[4];
+    for (int nu = 0; nu < 4; nu++) {
+      int d[4] = {0, 0, 0, 0};
+      d[nu] = 1;
+      ysp[nu] = site_of(c, src[0] + d[0], src[1] + d[1], src[2] + d[2], src[3] + d[3]);
+    }
+
+    Field up, dn, xu, xd, b12, x12;
+    double *cw = nullptr, *dqs = nullptr;
+    double2 *psi = nullptr, *K = nullptr, *blk = nullptr;
+    std::vector<double> conn((size_t)32 * V);
+    double contact[8];
+    cvchip_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, xu, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, xd, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, b12, 12, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, x12, 12, CVCHIP_PREC_F64);
+    auto hipok = [&](hipError_t e, const char *w) {
+      if (e != hipSuccess && !rc) rc = hip_fail(e, w, __FILE__, __LINE__);
+    };
+    if (!rc) {
+      hipok(hipMalloc(&cw, sizeof(double) * V), "weights");
+      hipok(hipMalloc(&dqs, sizeof(double) * (qs.size() + 4)), "momenta");
+      hipok(hipMalloc(&psi, sizeof(double2) * 288 * (size_t)V), "psi");
+      hipok(hipMalloc(&K, sizeof(double2) * 36 * (size_t)V), "force");
+      hipok(hipMalloc(&blk, sizeof(double2) * 720), "site block");
+    }
+    const int nblk_dn = (int)((V + 19) / 20);
+    if (!rc) rc = ensure_red(c, sizeof(double) * ((size_t)288 * nblk_dn + 64 * 256));
+    // ---- forward: propagators, conn, L
+    if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
+    if (!rc) rc = contract_fields(c, conn.data(), contact, up, dn, src);
+    SrcLinks hL;
+    if (!rc) rc = source_links(c, src, &hL);
+    if (!rc) {
+      hipok(hipMemcpyAsync(dqs, qs.data(), sizeof(double) * qs.size(), hipMemcpyHostToDevice, c->stream), "momenta");
+      hipLaunchKernelGGL(k_weight, dim3((V + 255) / 256), dim3(256), 0, c->stream, cw, (const double *)dqs, nq, A,
+                         src[0], src[1], src[2], src[3]);
+      hipok(hipMemsetAsync(xu.data, 0, xu.bytes, c->stream), "xi");
+      hipok(hipMemsetAsync(xd.data, 0, xd.bytes, c->stream), "xi");
+      hipok(hipMemsetAsync(K, 0, sizeof(double2) * 36 * (size_t)V, c->stream), "force");
+    }
+    std::vector<double> hcw(V);
+    if (!rc) {
+      hipok(hipMemcpyAsync(hcw.data(), cw, sizeof(double) * V, hipMemcpyDeviceToHost, c->stream), "weights");
+      hipok(hipStreamSynchronize(c->stream), "weights");
+    }
+    double Lval = 0.;
+    if (!rc) {
+      for (uint32_t x = 0; x < V; x++)
+        for (int mu = 0; mu < 4; mu++) Lval += -4.0 * hcw[x] * conn[2 * ((size_t)16 * x + 5 * mu)];
+      for (int mu = 0; mu < 4; mu++) Lval -= nq * contact[2 * mu];
+    }
+    // ---- adjoint sources, explicit sink force, source bilinears (per nu = mu)
+    cd Ksrc[4][9];
+    for (int nu = 0; nu < 4; nu++)
+      for (int q = 0; q < 9; q++) Ksrc[nu][q] = 0.;
+    std::vector<double> bil(288);
+    double *dres = nullptr;
+    if (!rc) hipok(hipMalloc(&dres, sizeof(double) * 288), "bilinears");
     for (int nu = 0; nu < 4 && !rc; nu++) {
-      hipLaunchKernelGGL(k_gather_site, dim3(3), dim3(256), 0, c->stream, blk, (const double2 *)up.data, V, ysp[nu]);
-      hipok(hipMemcpyAsync(su4.data(), blk, sizeof(double2) * 720, hipMemcpyDeviceToHost, c->stream), "site");
-      hipLaunchKernelGGL(k_gather_site, dim3(3), dim3(256), 0, c->stream, blk, (const double2 *)up.data, V, A.y);
-      hipok(hipMemcpyAsync(suy.data(), blk, sizeof(double2) * 720, hipMemcpyDeviceToHost, c->stream), "site");
-      hipok(hipStreamSynchronize(c->stream), "site");
+      rc = launch_psi(c, psi, dn, hL, nu);
       if (rc) break;
+      hipLaunchKernelGGL(k_xi_up, dim3((V * 12u + 255) / 256), dim3(256), 0, c->stream, (double2 *)xu.data,
+                         (const double2 *)psi, (const double *)cw, (const double2 *)c->gauge64, A, nu);
+      hipLaunchKernelGGL(k_xi_dn, dim3(nblk_dn), dim3(240), 0, c->stream, (double2 *)xd.data, (double *)c->red.ptr,
+                         (const double2 *)up.data, (const double2 *)dn.data, (const double *)cw,
+                         (const double2 *)c->gauge64, hL, A, nu);
+      hipok(hipGetLastError(), "adjoint sources");
+      if (!rc) rc = reduce_partials(c, (const double *)c->red.ptr, nblk_dn, 288, 1, dres);
+      hipLaunchKernelGGL(k_force_sink, dim3((V + 255) / 256), dim3(256), 0, c->stream, K, (const double2 *)up.data,
+                         (const double2 *)psi, (const double *)cw, (const double2 *)c->gauge64, A, nu);
+      hipok(hipGetLastError(), "sink force");
+      hipok(hipMemcpyAsync(bil.data(), dres, sizeof(double) * 288, hipMemcpyDeviceToHost, c->stream), "bilinears");
+      hipok(hipStreamSynchronize(c->stream), "bilinears");
+      if (rc) break;
+      // source-link derivative through the vertices: K += Us N1 - N2 Us^+
       const double2 *Us = hL.u[nu];
       cd U3[9];
       for (int q = 0; q < 9; q++) U3[q] = cd(Us[q].x, Us[q].y);
-      // Xi_u4,k(y+nu) += nQ/2 Us^+ (gamma_nu - 1) e_k ; Xi_unu,k(y) += -nQ/2 Us (gamma_nu + 1) e_k
-      std::vector<double2> add1(144), add2(144);
-      for (int k = 0; k < 12; k++) {
-        const int ir = k / 3, ia = k % 3;
-        for (int a = 0; a < 4; a++)
-          for (int col = 0; col < 3; col++) {
-            const cd gm = gmat(nu, a, ir) - (a == ir ? 1. : 0.);   // ((gamma_nu - 1) e_k)_a = (g - 1)_{a, ir}
-            const cd gp = gmat(nu, a, ir) + (a == ir ? 1. : 0.);
-            const cd v1 = 0.5 * nq * gm * std::conj(U3[3 * ia + col]);   // (Us^+)_{col, ia}
-            const cd v2 = -0.5 * nq * gp * U3[3 * col + ia];             // (Us)_{col, ia}
-            add1[(3 * a + col) * 12 + k] = make_double2(v1.real(), v1.imag());
-            add2[(3 * a + col) * 12 + k] = make_double2(v2.real(), v2.imag());
-          }
-      }
-      double2 *dadd = blk;   // reuse the 720-entry buffer (144 needed)
-      hipok(hipMemcpyAsync(dadd, add1.data(), sizeof(double2) * 144, hipMemcpyHostToDevice, c->stream), "contact");
-      hipLaunchKernelGGL(k_add_site_block, dim3(1), dim3(256), 0, c->stream, (double2 *)xu.data,
-                         (const double2 *)dadd, V, ysp[nu], 48);
-      hipok(hipStreamSynchronize(c->stream), "contact");
-      hipok(hipMemcpyAsync(dadd, add2.data(), sizeof(double2) * 144, hipMemcpyHostToDevice, c->stream), "contact");
-      hipLaunchKernelGGL(k_add_site_block, dim3(1), dim3(256), 0, c->stream, (double2 *)xu.data,
-                         (const double2 *)dadd, V, A.y, 12 * nu);
-      hipok(hipStreamSynchronize(c->stream), "contact");
-      // explicit: K_src += nQ/2 (C1 + C2)
-      //   C1_{j,ia} = sum_{ir,a} (g-1)_{ir,a} X_{(ir,ia)}[a][j],  X_k = Us Su4_k(y+nu)
-      //   C2_{d,c}  = sum_{k,a} (g+1)_{ir,a} Y_k[a][d] (Us^+)_{ia,c},  Y_k = Sunu_k(y)
-      cd C1[9], C2[9];
-      for (int q = 0; q < 9; q++) C1[q] = C2[q] = 0.;
-      for (int k = 0; k < 12; k++) {
-        const int ir = k / 3, ia = k % 3;
-        for (int a = 0; a < 4; a++) {
-          const cd gm = gmat(nu, ir, a) - (a == ir ? 1. : 0.);
-          const cd gp = gmat(nu, ir, a) + (a == ir ? 1. : 0.);
-          for (int jj = 0; jj < 3; jj++) {
-            cd x = 0.;
-            for (int m = 0; m < 3; m++) {
-              const double2 s4 = su4[(3 * a + m) * 60 + 48 + k];
-              x += U3[3 * jj + m] * cd(s4.x, s4.y);
-            }
-            C1[3 * jj + ia] += gm * x;
-            const double2 sy = suy[(3 * a + jj) * 60 + 12 * nu + k];
-            for (int cc = 0; cc < 3; cc++) C2[3 * jj + cc] += gp * cd(sy.x, sy.y) * std::conj(U3[3 * cc + ia]);
+      cd N1[9], N2[9];
+      for (int q = 0; q < 9; q++) N1[q] = N2[q] = 0.;
+      for (int l = 0; l < 12; l++) {
+        const int j = l / 3, ib = l % 3;
+        const int ira = GCOL[nu][j];
+        const cd ea = gmat(nu, ira, j) * g5d(j);
+        const double s5 = g5d(j);
+        for (int w = 0; w < 2; w++) {
+          const cd e1 = w == 0 ? ea : cd(-s5, 0), e2 = w == 0 ? ea : cd(s5, 0);
+          for (int ia = 0; ia < 3; ia++) {
+            const cd b1(bil[l * 24 + 2 * (3 * w + ia)], bil[l * 24 + 2 * (3 * w + ia) + 1]);
+            const cd b2(bil[l * 24 + 12 + 2 * (3 * w + ia)], bil[l * 24 + 12 + 2 * (3 * w + ia) + 1]);
+            N1[3 * ib + ia] += e1 * b1;
+            N2[3 * ib + ia] += e2 * b2;
           }
         }
       }
-      for (int q = 0; q < 9; q++) Ksrc[nu][q] += 0.5 * nq * (C1[q] + C2[q]);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          cd s1 = 0., s2 = 0.;
+          for (int m = 0; m < 3; m++) {
+            s1 += U3[3 * a + m] * N1[3 * m + b];
+            s2 += N2[3 * a + m] * std::conj(U3[3 * b + m]);
+          }
+          Ksrc[nu][3 * a + b] += s1 - s2;
+        }
     }
-  }
-  // ---- adjoint solves: Lambda_up = g5 D(-mu)^-1 g5 Xi_up, Lambda_dn = g5 D(mu)^-1 g5 Xi_dn (in place)
-  const double mu0 = c->mu;
-  for (int f = 0; f < 2 && !rc; f++) {
-    Field &xi = f == 0 ? xu : xd;
-    for (int s = 0; s < 5 && !rc; s++) {
-      hipLaunchKernelGGL(k_cols, dim3((144u * V + 255) / 256), dim3(256), 0, c->stream, (double2 *)b12.data,
-                         (const double2 *)xi.data, V, 0, 12 * s, 1);
-      hipok(hipMemsetAsync(x12.data, 0, x12.bytes, c->stream), "adjoint");
-      c->mu = f == 0 ? -mu0 : mu0;
-      cvchip_stats st;
-      memset(&st, 0, sizeof(st));
-      if (!rc) rc = invert_her_fields(c, x12, b12, sp, niter_max, prec, delta, 0, &st);
-      c->mu = mu0;
-      if (rc > 0) rc = 0;
-      if (rc) break;
-      tot.iterations += st.iterations;
-      tot.converged += st.converged;
-      tot.reliable_updates += st.reliable_updates;
-      tot.dslash_calls += st.dslash_calls;
-      tot.seconds += st.seconds;
-      hipLaunchKernelGGL(k_cols, dim3((144u * V + 255) / 256), dim3(256), 0, c->stream, (double2 *)xi.data,
-                         (const double2 *)x12.data, V, 1, 12 * s, 1);
-      hipok(hipGetLastError(), "adjoint columns");
-    }
-  }
-  // ---- dL/dmu = Im<Lambda_up, g5 S_up> - Im<Lambda_dn, g5 S_dn>
-  double dmu = 0.;
-  if (!rc) {
-    const uint32_t VN = V * 60u;
-    const int nb = (int)((VN + 255) / 256);
-    rc = ensure_red(c, sizeof(double) * ((size_t)2 * nb + 64 * 256));
-    double r2[2][2];
-    for (int f = 0; f < 2 && !rc; f++) {
-      hipLaunchKernelGGL(k_dot_g5, dim3(nb), dim3(256), 0, c->stream, (double *)c->red.ptr,
-                         (const double2 *)(f == 0 ? xu.data : xd.data), (const double2 *)(f == 0 ? up.data : dn.data),
-                         VN);
-      if (!rc) rc = reduce_partials(c, (const double *)c->red.ptr, nb, 2, 1, dres);
-      hipok(hipMemcpyAsync(r2[f], dres, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream), "dmu");
-      hipok(hipStreamSynchronize(c->stream), "dmu");
-    }
-    dmu = r2[0][1] - r2[1][1];
-  }
-  // ---- implicit link force and projection
-  if (!rc && force) {
-    hipLaunchKernelGGL(k_force_impl, dim3((V * 4u + 255) / 256), dim3(256), 0, c->stream, K, (const double2 *)up.data,
-                       (const double2 *)dn.data, (const double2 *)xu.data, (const double2 *)xd.data,
-                       (const double2 *)c->gauge64, A);
-    hipok(hipGetLastError(), "force");
-    std::vector<double2> hK((size_t)36 * V);
-    if (!rc) rc = copy_to_host(c, hK.data(), K, sizeof(double2) * 36 * (size_t)V);
+    // ---- contact term: adjoint sources at y+nu (up from y) and y (up from y+nu), explicit source-link part
     if (!rc) {
-      for (int nu = 0; nu < 4; nu++)
-        for (int q = 0; q < 9; q++) {
-          double2 &e = hK[((size_t)A.y * 4 + nu) * 9 + q];
-          e.x += Ksrc[nu][q].real();
-          e.y += Ksrc[nu][q].imag();
+      std::vector<double2> su4(720), suy(720);
+      for (int nu = 0; nu < 4 && !rc; nu++) {
+        hipLaunchKernelGGL(k_gather_site, dim3(3), dim3(256), 0, c->stream, blk, (const double2 *)up.data, V, ysp[nu]);
+        hipok(hipMemcpyAsync(su4.data(), blk, sizeof(double2) * 720, hipMemcpyDeviceToHost, c->stream), "site");
+        hipLaunchKernelGGL(k_gather_site, dim3(3), dim3(256), 0, c->stream, blk, (const double2 *)up.data, V, A.y);
+        hipok(hipMemcpyAsync(suy.data(), blk, sizeof(double2) * 720, hipMemcpyDeviceToHost, c->stream), "site");
+        hipok(hipStreamSynchronize(c->stream), "site");
+        if (rc) break;
+        const double2 *Us = hL.u[nu];
+        cd U3[9];
+        for (int q = 0; q < 9; q++) U3[q] = cd(Us[q].x, Us[q].y);
+        // Xi_u4,k(y+nu) += nQ/2 Us^+ (gamma_nu - 1) e_k ; Xi_unu,k(y) += -nQ/2 Us (gamma_nu + 1) e_k
+        std::vector<double2> add1(144), add2(144);
+        for (int k = 0; k < 12; k++) {
+          const int ir = k / 3, ia = k % 3;
+          for (int a = 0; a < 4; a++)
+            for (int col = 0; col < 3; col++) {
+              const cd gm = gmat(nu, a, ir) - (a == ir ? 1. : 0.);   // ((gamma_nu - 1) e_k)_a = (g - 1)_{a, ir}
+              const cd gp = gmat(nu, a, ir) + (a == ir ? 1. : 0.);
+              const cd v1 = 0.5 * nq * gm * std::conj(U3[3 * ia + col]);   // (Us^+)_{col, ia}
+              const cd v2 = -0.5 * nq * gp * U3[3 * col + ia];             // (Us)_{col, ia}
+              add1[(3 * a + col) * 12 + k] = make_double2(v1.real(), v1.imag());
+              add2[(3 * a + col) * 12 + k] = make_double2(v2.real(), v2.imag());
+            }
         }
-      for (size_t zl = 0; zl < (size_t)4 * V; zl++) {
-        cd Km[9], F[9];
-        for (int q = 0; q < 9; q++) Km[q] = cd(hK[zl * 9 + q].x, hK[zl * 9 + q].y);
-        for (int a = 0; a < 3; a++)
-          for (int b = 0; b < 3; b++) F[3 * a + b] = cd(0, 0.5) * (Km[3 * a + b] - std::conj(Km[3 * b + a]));
-        const cd tr = (F[0] + F[4] + F[8]) / 3.0;
-        F[0] -= tr;
-        F[4] -= tr;
-        F[8] -= tr;
-        for (int q = 0; q < 9; q++) {
-          force[zl * 18 + 2 * q] = F[q].real();
-          force[zl * 18 + 2 * q + 1] = F[q].imag();
+        double2 *dadd = blk;   // reuse the 720-entry buffer (144 needed)
+        hipok(hipMemcpyAsync(dadd, add1.data(), sizeof(double2) * 144, hipMemcpyHostToDevice, c->stream), "contact");
+        hipLaunchKernelGGL(k_add_site_block, dim3(1), dim3(256), 0, c->stream, (double2 *)xu.data,
+                           (const double2 *)dadd, V, ysp[nu], 48);
+        hipok(hipStreamSynchronize(c->stream), "contact");
+        hipok(hipMemcpyAsync(dadd, add2.data(), sizeof(double2) * 144, hipMemcpyHostToDevice, c->stream), "contact");
+        hipLaunchKernelGGL(k_add_site_block, dim3(1), dim3(256), 0, c->stream, (double2 *)xu.data,
+                           (const double2 *)dadd, V, A.y, 12 * nu);
+        hipok(hipStreamSynchronize(c->stream), "contact");
+        // explicit: K_src += nQ/2 (C1 + C2)
+        //   C1_{j,ia} = sum_{ir,a} (g-1)_{ir,a} X_{(ir,ia)}[a][j],  X_k = Us Su4_k(y+nu)
+        //   C2_{d,c}  = sum_{k,a} (g+1)_{ir,a} Y_k[a][d] (Us^+)_{ia,c},  Y_k = Sunu_k(y)
+        cd C1[9], C2[9];
+        for (int q = 0; q < 9; q++) C1[q] = C2[q] = 0.;
+        for (int k = 0; k < 12; k++) {
+          const int ir = k / 3, ia = k % 3;
+          for (int a = 0; a < 4; a++) {
+            const cd gm = gmat(nu, ir, a) - (a == ir ? 1. : 0.);
+            const cd gp = gmat(nu, ir, a) + (a == ir ? 1. : 0.);
+            for (int jj = 0; jj < 3; jj++) {
+              cd x = 0.;
+              for (int m = 0; m < 3; m++) {
+                const double2 s4 = su4[(3 * a + m) * 60 + 48 + k];
+                x += U3[3 * jj + m] * cd(s4.x, s4.y);
+              }
+              C1[3 * jj + ia] += gm * x;
+              const double2 sy = suy[(3 * a + jj) * 60 + 12 * nu + k];
+              for (int cc = 0; cc < 3; cc++) C2[3 * jj + cc] += gp * cd(sy.x, sy.y) * std::conj(U3[3 * cc + ia]);
+            }
+          }
+        }
+        for (int q = 0; q < 9; q++) Ksrc[nu][q] += 0.5 * nq * (C1[q] + C2[q]);
+      }
+    }
+    // ---- adjoint solves: Lambda_up = g5 D(-mu)^-1 g5 Xi_up, Lambda_dn = g5 D(mu)^-1 g5 Xi_dn (in place)
+    const double mu0 = c->mu;
+    for (int f = 0; f < 2 && !rc; f++) {
+      Field &xi = f == 0 ? xu : xd;
+      for (int s = 0; s < 5 && !rc; s++) {
+        hipLaunchKernelGGL(k_cols, dim3((144u * V + 255) / 256), dim3(256), 0, c->stream, (double2 *)b12.data,
+                           (const double2 *)xi.data, V, 0, 12 * s, 1);
+        hipok(hipMemsetAsync(x12.data, 0, x12.bytes, c->stream), "adjoint");
+        c->mu = f == 0 ? -mu0 : mu0;
+        cvchip_stats st;
+        memset(&st, 0, sizeof(st));
+        if (!rc) rc = invert_her_fields(c, x12, b12, sp, niter_max, prec, delta, 0, &st);
+        c->mu = mu0;
+        if (rc > 0) rc = 0;
+        if (rc) break;
+        tot.iterations += st.iterations;
+        tot.converged += st.converged;
+        tot.reliable_updates += st.reliable_updates;
+        tot.dslash_calls += st.dslash_calls;
+        tot.seconds += st.seconds;
+        hipLaunchKernelGGL(k_cols, dim3((144u * V + 255) / 256), dim3(256), 0, c->stream, (double2 *)xi.data,
+                           (const double2 *)x12.data, V, 1, 12 * s, 1);
+        hipok(hipGetLastError(), "adjoint columns");
+      }
+    }
+    // ---- dL/dmu = Im<Lambda_up, g5 S_up> - Im<Lambda_dn, g5 S_dn>
+    double dmu = 0.;
+    if (!rc) {
+      const uint32_t VN = V * 60u;
+      const int nb = (int)((VN + 255) / 256);
+      rc = ensure_red(c, sizeof(double) * ((size_t)2 * nb + 64 * 256));
+      double r2[2][2];
+      for (int f = 0; f < 2 && !rc; f++) {
+        hipLaunchKernelGGL(k_dot_g5, dim3(nb), dim3(256), 0, c->stream, (double *)c->red.ptr,
+                           (const double2 *)(f == 0 ? xu.data : xd.data), (const double2 *)(f == 0 ? up.data : dn.data),
+                           VN);
+        if (!rc) rc = reduce_partials(c, (const double *)c->red.ptr, nb, 2, 1, dres);
+        hipok(hipMemcpyAsync(r2[f], dres, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream), "dmu");
+        hipok(hipStreamSynchronize(c->stream), "dmu");
+      }
+      dmu = r2[0][1] - r2[1][1];
+    }
+    // ---- implicit link force and projection
+    if (!rc && force) {
+      hipLaunchKernelGGL(k_force_impl, dim3((V * 4u + 255) / 256), dim3(256), 0, c->stream, K, (const double2 *)up.data,
+                         (const double2 *)dn.data, (const double2 *)xu.data, (const double2 *)xd.data,
+                         (const double2 *)c->gauge64, A);
+      hipok(hipGetLastError(), "force");
+      std::vector<double2> hK((size_t)36 * V);
+      if (!rc) rc = copy_to_host(c, hK.data(), K, sizeof(double2) * 36 * (size_t)V);
+      if (!rc) {
+        for (int nu = 0; nu < 4; nu++)
+          for (int q = 0; q < 9; q++) {
+            double2 &e = hK[((size_t)A.y * 4 + nu) * 9 + q];
+            e.x += Ksrc[nu][q].real();
+            e.y += Ksrc[nu][q].imag();
+          }
+        for (size_t zl = 0; zl < (size_t)4 * V; zl++) {
+          cd Km[9], F[9];
+          for (int q = 0; q < 9; q++) Km[q] = cd(hK[zl * 9 + q].x, hK[zl * 9 + q].y);
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) F[3 * a + b] = cd(0, 0.5) * (Km[3 * a + b] - std::conj(Km[3 * b + a]));
+          const cd tr = (F[0] + F[4] + F[8]) / 3.0;
+          F[0] -= tr;
+          F[4] -= tr;
+          F[8] -= tr;
+          for (int q = 0; q < 9; q++) {
+            force[zl * 18 + 2 * q] = F[q].real();
+            force[zl * 18 + 2 * q + 1] = F[q].imag();
+          }
         }
       }
     }
-  }
-  *L_out = Lval;
-  *dmu_out = dmu;
-  if (stats) *stats = tot;
-  field_free(up);
-  field_free(dn);
-  field_free(xu);
-  field_free(xd);
-  field_free(b12);
-  field_free(x12);
-  if (cw) (void)hipFree(cw);
-  if (dqs) (void)hipFree(dqs);
-  if (psi) (void)hipFree(psi);
-  if (K) (void)hipFree(K);
-  if (blk) (void)hipFree(blk);
-  if (dres) (void)hipFree(dres);
-  return rc;
+    *L_out = Lval;
+    *dmu_out = dmu;
+    if (stats) *stats = tot;
+    field_free(up);
+    field_free(dn);
+    field_free(xu);
+    field_free(xd);
+    field_free(b12);
+    field_free(x12);
+    if (cw) (void)hipFree(cw);
+    if (dqs) (void)hipFree(dqs);
+    if (psi) (void)hipFree(psi);
+    if (K) (void)hipFree(K);
+    if (blk) (void)hipFree(blk);
+    if (dres) (void)hipFree(dres);
+    return rc;
+  });
 }

@@ -502,76 +502,84 @@ using namespace cvchip;
 
 extern "C" int cvchip_field_cvc_contract(cvchip_ctx *c, double *conn, double contact[8],
                                          const cvchip_field *up, const cvchip_field *dn, const int src[4]) {
-  if (!c || !conn || !contact || !up || !dn || !src) {
-    set_error("NULL argument to cvchip_field_cvc_contract");
-    return CVCHIP_ERR_NULL;
-  }
-  return contract_fields(c, conn, contact, up->f, dn->f, src);
+  return cvchip::guarded("cvchip_field_cvc_contract", [&]() -> int {
+    if (!c || !conn || !contact || !up || !dn || !src) {
+      set_error("NULL argument to cvchip_field_cvc_contract");
+      return CVCHIP_ERR_NULL;
+    }
+    return contract_fields(c, conn, contact, up->f, dn->f, src);
+  });
 }
 
 extern "C" int cvchip_cvc_contract(cvchip_ctx *c, double *conn, double contact[8], const double *const *props,
                                    const int src[4]) {
-  if (!c || !conn || !contact || !props || !src) {
-    set_error("NULL argument to cvchip_cvc_contract");
-    return CVCHIP_ERR_NULL;
-  }
-  for (int k = 0; k < 120; k++)
-    if (!props[k]) {
-      set_error("propagator %d is NULL", k);
+  return cvchip::guarded("cvchip_cvc_contract", [&]() -> int {
+    if (!c || !conn || !contact || !props || !src) {
+      set_error("NULL argument to cvchip_cvc_contract");
       return CVCHIP_ERR_NULL;
     }
-  Field up, dn;
-  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
-  for (int k = 0; k < 120 && !rc; k++) rc = upload_aos(c, k < 60 ? up : dn, props[k], k % 60, 1);
-  if (!rc) rc = contract_fields(c, conn, contact, up, dn, src);
-  field_free(up);
-  field_free(dn);
-  return rc;
+    for (int k = 0; k < 120; k++)
+      if (!props[k]) {
+        set_error("propagator %d is NULL", k);
+        return CVCHIP_ERR_NULL;
+      }
+    Field up, dn;
+    int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+    for (int k = 0; k < 120 && !rc; k++) rc = upload_aos(c, k < 60 ? up : dn, props[k], k % 60, 1);
+    if (!rc) rc = contract_fields(c, conn, contact, up, dn, src);
+    field_free(up);
+    field_free(dn);
+    return rc;
+  });
 }
 
 extern "C" int cvchip_wi(cvchip_ctx *c, double *wi, const double *conn) {
-  if (!c || !wi || !conn) {
-    set_error("NULL argument to cvchip_wi");
-    return CVCHIP_ERR_NULL;
-  }
-  const size_t V = (size_t)c->lat.V;
-  double2 *dc = nullptr, *dw = nullptr;
-  CVC_HIP(hipMalloc(&dc, sizeof(double2) * 16 * V));
-  hipError_t e = hipMalloc(&dw, sizeof(double2) * 4 * V);
-  int rc = e == hipSuccess ? 0 : hip_fail(e, "Ward identity buffers", __FILE__, __LINE__);
-  if (!rc) rc = copy_to_device(c, dc, conn, sizeof(double2) * 16 * V);
-  if (!rc) {
-    hipLaunchKernelGGL(k_wi, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c->stream, dw, (const double2 *)dc,
-                       conn_params(c));
-    e = hipGetLastError();
-    if (e != hipSuccess) rc = hip_fail(e, "Ward identity", __FILE__, __LINE__);
-  }
-  if (!rc) rc = copy_to_host(c, wi, dw, sizeof(double2) * 4 * V);
-  (void)hipStreamSynchronize(c->stream);   // nothing in flight before the buffers go
-  (void)hipFree(dc);
-  if (dw) (void)hipFree(dw);
-  return rc;
+  return cvchip::guarded("cvchip_wi", [&]() -> int {
+    if (!c || !wi || !conn) {
+      set_error("NULL argument to cvchip_wi");
+      return CVCHIP_ERR_NULL;
+    }
+    const size_t V = (size_t)c->lat.V;
+    double2 *dc = nullptr, *dw = nullptr;
+    CVC_HIP(hipMalloc(&dc, sizeof(double2) * 16 * V));
+    hipError_t e = hipMalloc(&dw, sizeof(double2) * 4 * V);
+    int rc = e == hipSuccess ? 0 : hip_fail(e, "Ward identity buffers", __FILE__, __LINE__);
+    if (!rc) rc = copy_to_device(c, dc, conn, sizeof(double2) * 16 * V);
+    if (!rc) {
+      hipLaunchKernelGGL(k_wi, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c->stream, dw, (const double2 *)dc,
+                         conn_params(c));
+      e = hipGetLastError();
+      if (e != hipSuccess) rc = hip_fail(e, "Ward identity", __FILE__, __LINE__);
+    }
+    if (!rc) rc = copy_to_host(c, wi, dw, sizeof(double2) * 4 * V);
+    (void)hipStreamSynchronize(c->stream);   // nothing in flight before the buffers go
+    (void)hipFree(dc);
+    if (dw) (void)hipFree(dw);
+    return rc;
+  });
 }
 
 extern "C" int cvchip_pi_q(cvchip_ctx *c, double *piq, const double *conn, const double contact[8],
                            const int src[4]) {
-  if (!c || !piq || !conn || !contact || !src) {
-    set_error("NULL argument to cvchip_pi_q");
-    return CVCHIP_ERR_NULL;
-  }
-  const size_t V = (size_t)c->lat.V;
-  double2 *a = nullptr, *t = nullptr;
-  CVC_HIP(hipMalloc(&a, sizeof(double2) * 16 * V));
-  hipError_t e = hipMalloc(&t, sizeof(double2) * 16 * V);
-  int rc = e == hipSuccess ? 0 : hip_fail(e, "Pi(q) buffers", __FILE__, __LINE__);
-  if (!rc) rc = copy_to_device(c, a, conn, sizeof(double2) * 16 * V);
-  if (!rc) rc = pi_q_device(c, a, t, contact, src);
-  if (!rc) rc = copy_to_host(c, piq, a, sizeof(double2) * 16 * V);
-  (void)hipStreamSynchronize(c->stream);   // nothing in flight before the buffers go
-  (void)hipFree(a);
-  if (t) (void)hipFree(t);
-  return rc;
+  return cvchip::guarded("cvchip_pi_q", [&]() -> int {
+    if (!c || !piq || !conn || !contact || !src) {
+      set_error("NULL argument to cvchip_pi_q");
+      return CVCHIP_ERR_NULL;
+    }
+    const size_t V = (size_t)c->lat.V;
+    double2 *a = nullptr, *t = nullptr;
+    CVC_HIP(hipMalloc(&a, sizeof(double2) * 16 * V));
+    hipError_t e = hipMalloc(&t, sizeof(double2) * 16 * V);
+    int rc = e == hipSuccess ? 0 : hip_fail(e, "Pi(q) buffers", __FILE__, __LINE__);
+    if (!rc) rc = copy_to_device(c, a, conn, sizeof(double2) * 16 * V);
+    if (!rc) rc = pi_q_device(c, a, t, contact, src);
+    if (!rc) rc = copy_to_host(c, piq, a, sizeof(double2) * 16 * V);
+    (void)hipStreamSynchronize(c->stream);   // nothing in flight before the buffers go
+    (void)hipFree(a);
+    if (t) (void)hipFree(t);
+    return rc;
+  });
 }
 
 namespace cvchip {
@@ -625,44 +633,48 @@ int solve_source_props(cvchip_ctx *c, const int src[4], double sp, int niter_max
 // contraction.
 extern "C" int cvchip_cvc_source(cvchip_ctx *c, double *conn, double contact[8], const int src[4], double sp,
                                  int niter_max, int prec, double delta, cvchip_stats *stats) {
-  if (!c || !conn || !contact || !src) {
-    set_error("NULL argument to cvchip_cvc_source");
-    return CVCHIP_ERR_NULL;
-  }
-  Field up, dn;
-  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
-  cvchip_stats tot;
-  memset(&tot, 0, sizeof(tot));
-  if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
-  if (!rc) rc = contract_fields(c, conn, contact, up, dn, src);
-  if (stats) *stats = tot;
-  field_free(up);
-  field_free(dn);
-  return rc;
+  return cvchip::guarded("cvchip_cvc_source", [&]() -> int {
+    if (!c || !conn || !contact || !src) {
+      set_error("NULL argument to cvchip_cvc_source");
+      return CVCHIP_ERR_NULL;
+    }
+    Field up, dn;
+    int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+    cvchip_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
+    if (!rc) rc = contract_fields(c, conn, contact, up, dn, src);
+    if (stats) *stats = tot;
+    field_free(up);
+    field_free(dn);
+    return rc;
+  });
 }
 
 // The propagators of cvchip_cvc_source, downloaded into the reference layout.
 extern "C" int cvchip_cvc_propagators(cvchip_ctx *c, double *const *props, const int src[4], double sp, int niter_max,
                                       int prec, double delta, cvchip_stats *stats) {
-  if (!c || !props || !src) {
-    set_error("NULL argument to cvchip_cvc_propagators");
-    return CVCHIP_ERR_NULL;
-  }
-  for (int k = 0; k < 120; k++)
-    if (!props[k]) {
-      set_error("propagator %d is NULL", k);
+  return cvchip::guarded("cvchip_cvc_propagators", [&]() -> int {
+    if (!c || !props || !src) {
+      set_error("NULL argument to cvchip_cvc_propagators");
       return CVCHIP_ERR_NULL;
     }
-  Field up, dn;
-  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
-  cvchip_stats tot;
-  memset(&tot, 0, sizeof(tot));
-  if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
-  for (int k = 0; k < 120 && !rc; k++) rc = download_aos(c, k < 60 ? up : dn, props[k], k % 60, 1);
-  if (stats) *stats = tot;
-  field_free(up);
-  field_free(dn);
-  return rc;
+    for (int k = 0; k < 120; k++)
+      if (!props[k]) {
+        set_error("propagator %d is NULL", k);
+        return CVCHIP_ERR_NULL;
+      }
+    Field up, dn;
+    int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+    cvchip_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
+    for (int k = 0; k < 120 && !rc; k++) rc = download_aos(c, k < 60 ? up : dn, props[k], k % 60, 1);
+    if (stats) *stats = tot;
+    field_free(up);
+    field_free(dn);
+    return rc;
+  });
 }

@@ -360,262 +360,284 @@ using namespace cvchip;
 // =================================================================== C ABI
 
 extern "C" int cvchip_field_zero(cvchip_ctx *c, cvchip_field *fh) {
-  if (!c || !fh) {
-    set_error("NULL argument to cvchip_field_zero");
-    return CVCHIP_ERR_NULL;
-  }
-  Field &f = fh->f;
-  const uint64_t n = f.bytes / 16;   // every set is a whole number of 16-byte planes
-  if (n) hipLaunchKernelGGL(k_zero16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (uint4 *)f.data, n);
-  if (f.norm) {
-    const uint64_t m = (uint64_t)f.V * f.nrhs;   // one float per (site, rhs): zero scale
-    hipLaunchKernelGGL(k_zero16, dim3((unsigned)((m / 4 + 255) / 256)), dim3(256), 0, c->stream, (uint4 *)f.norm, m / 4);
-    if (m % 4) CVC_HIP(hipMemsetAsync(f.norm + (m / 4) * 4, 0, sizeof(float) * (m % 4), c->stream));
-  }
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_zero", [&]() -> int {
+    if (!c || !fh) {
+      set_error("NULL argument to cvchip_field_zero");
+      return CVCHIP_ERR_NULL;
+    }
+    Field &f = fh->f;
+    const uint64_t n = f.bytes / 16;   // every set is a whole number of 16-byte planes
+    if (n) hipLaunchKernelGGL(k_zero16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (uint4 *)f.data, n);
+    if (f.norm) {
+      const uint64_t m = (uint64_t)f.V * f.nrhs;   // one float per (site, rhs): zero scale
+      hipLaunchKernelGGL(k_zero16, dim3((unsigned)((m / 4 + 255) / 256)), dim3(256), 0, c->stream, (uint4 *)f.norm, m / 4);
+      if (m % 4) CVC_HIP(hipMemsetAsync(f.norm + (m / 4) * 4, 0, sizeof(float) * (m % 4), c->stream));
+    }
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_point_sources(cvchip_ctx *c, cvchip_field *fh, int nsrc, const int *coords) {
-  if (!c || !fh || !coords) {
-    set_error("NULL argument to cvchip_field_point_sources");
-    return CVCHIP_ERR_NULL;
-  }
-  Field &f = fh->f;
-  if (f.prec != CVCHIP_PREC_F64 || f.V != c->lat.V || nsrc < 1 || nsrc > kMaxPointSources || f.nrhs != 12 * nsrc) {
-    set_error("point sources need a full-lattice FP64 set of 12 x nsrc columns, 1 <= nsrc <= %d (nsrc %d, %d columns)",
-              kMaxPointSources, nsrc, f.nrhs);
-    return CVCHIP_ERR_ARG;
-  }
-  const int L[4] = {c->lat.T, c->lat.LX, c->lat.LY, c->lat.LZ};
-  PointSites ps{};
-  for (int k = 0; k < nsrc; k++) {
-    const int *x = coords + 4 * k;
-    for (int m = 0; m < 4; m++)
-      if (x[m] < 0 || x[m] >= L[m]) {
-        set_error("source %d coordinate %d = %d outside 0..%d", k, m, x[m], L[m] - 1);
-        return CVCHIP_ERR_ARG;
-      }
-    ps.site[k] = (uint32_t)((((int64_t)x[0] * L[1] + x[1]) * L[2] + x[2]) * L[3] + x[3]);
-  }
-  const uint64_t n = (uint64_t)12 * f.V * f.nrhs;
-  hipLaunchKernelGGL(k_point_sources_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     (double2 *)f.data, (uint32_t)f.V, (uint32_t)f.nrhs, ps);
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_point_sources", [&]() -> int {
+    if (!c || !fh || !coords) {
+      set_error("NULL argument to cvchip_field_point_sources");
+      return CVCHIP_ERR_NULL;
+    }
+    Field &f = fh->f;
+    if (f.prec != CVCHIP_PREC_F64 || f.V != c->lat.V || nsrc < 1 || nsrc > kMaxPointSources || f.nrhs != 12 * nsrc) {
+      set_error("point sources need a full-lattice FP64 set of 12 x nsrc columns, 1 <= nsrc <= %d (nsrc %d, %d columns)",
+                kMaxPointSources, nsrc, f.nrhs);
+      return CVCHIP_ERR_ARG;
+    }
+    const int L[4] = {c->lat.T, c->lat.LX, c->lat.LY, c->lat.LZ};
+    PointSites ps{};
+    for (int k = 0; k < nsrc; k++) {
+      const int *x = coords + 4 * k;
+      for (int m = 0; m < 4; m++)
+        if (x[m] < 0 || x[m] >= L[m]) {
+          set_error("source %d coordinate %d = %d outside 0..%d", k, m, x[m], L[m] - 1);
+          return CVCHIP_ERR_ARG;
+        }
+      ps.site[k] = (uint32_t)((((int64_t)x[0] * L[1] + x[1]) * L[2] + x[2]) * L[3] + x[3]);
+    }
+    const uint64_t n = (uint64_t)12 * f.V * f.nrhs;
+    hipLaunchKernelGGL(k_point_sources_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       (double2 *)f.data, (uint32_t)f.V, (uint32_t)f.nrhs, ps);
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_copy_columns(cvchip_ctx *c, cvchip_field *dst, int dst_col0, const cvchip_field *src,
                                          int src_col0, int n) {
-  if (!c || !dst || !src) {
-    set_error("NULL argument to cvchip_field_copy_columns");
-    return CVCHIP_ERR_NULL;
-  }
-  const Field &d = dst->f, &s = src->f;
-  if (d.prec != CVCHIP_PREC_F64 || s.prec != CVCHIP_PREC_F64 || d.V != s.V) {
-    set_error("column copies need FP64 sets of the same volume");
-    return CVCHIP_ERR_ARG;
-  }
-  if (n < 0 || dst_col0 < 0 || src_col0 < 0 || dst_col0 + n > d.nrhs || src_col0 + n > s.nrhs ||
-      (d.data == s.data && dst_col0 < src_col0 + n && src_col0 < dst_col0 + n)) {
-    set_error("column ranges %d+%d of %d and %d+%d of %d are invalid or overlap", dst_col0, n, d.nrhs, src_col0, n,
-              s.nrhs);
-    return CVCHIP_ERR_ARG;
-  }
-  if (n == 0) return CVCHIP_OK;
-  const uint64_t tot = (uint64_t)12 * d.V * n;
-  hipLaunchKernelGGL(k_copy_columns, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, (double2 *)d.data,
-                     (const double2 *)s.data, (uint32_t)d.V, (uint32_t)d.nrhs, (uint32_t)s.nrhs, (uint32_t)dst_col0,
-                     (uint32_t)src_col0, (uint32_t)n);
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_copy_columns", [&]() -> int {
+    if (!c || !dst || !src) {
+      set_error("NULL argument to cvchip_field_copy_columns");
+      return CVCHIP_ERR_NULL;
+    }
+    const Field &d = dst->f, &s = src->f;
+    if (d.prec != CVCHIP_PREC_F64 || s.prec != CVCHIP_PREC_F64 || d.V != s.V) {
+      set_error("column copies need FP64 sets of the same volume");
+      return CVCHIP_ERR_ARG;
+    }
+    if (n < 0 || dst_col0 < 0 || src_col0 < 0 || dst_col0 + n > d.nrhs || src_col0 + n > s.nrhs ||
+        (d.data == s.data && dst_col0 < src_col0 + n && src_col0 < dst_col0 + n)) {
+      set_error("column ranges %d+%d of %d and %d+%d of %d are invalid or overlap", dst_col0, n, d.nrhs, src_col0, n,
+                s.nrhs);
+      return CVCHIP_ERR_ARG;
+    }
+    if (n == 0) return CVCHIP_OK;
+    const uint64_t tot = (uint64_t)12 * d.V * n;
+    hipLaunchKernelGGL(k_copy_columns, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, (double2 *)d.data,
+                       (const double2 *)s.data, (uint32_t)d.V, (uint32_t)d.nrhs, (uint32_t)s.nrhs, (uint32_t)dst_col0,
+                       (uint32_t)src_col0, (uint32_t)n);
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_fill_random(cvchip_ctx *c, cvchip_field *fh, uint64_t seed) {
-  if (!c || !fh) return CVCHIP_ERR_NULL;
-  Field &f = fh->f;
-  if (f.prec != CVCHIP_PREC_F64) {
-    set_error("random fill is defined for FP64 sets (convert with cvchip_field_copy)");
-    return CVCHIP_ERR_ARG;
-  }
-  const uint64_t n = (uint64_t)12 * f.V * f.nrhs;
-  hipLaunchKernelGGL(k_fill_random, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (double2 *)f.data,
-                     n, seed);
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_fill_random", [&]() -> int {
+    if (!c || !fh) return CVCHIP_ERR_NULL;
+    Field &f = fh->f;
+    if (f.prec != CVCHIP_PREC_F64) {
+      set_error("random fill is defined for FP64 sets (convert with cvchip_field_copy)");
+      return CVCHIP_ERR_ARG;
+    }
+    const uint64_t n = (uint64_t)12 * f.V * f.nrhs;
+    hipLaunchKernelGGL(k_fill_random, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (double2 *)f.data,
+                       n, seed);
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" const char *cvchip_last_error(void) { return g_err; }
 
 extern "C" int cvchip_ctx_create(cvchip_ctx **ctx, const int dims[4], const double bc_angle[4],
                                  double kappa, double mu) {
-  if (!ctx || !dims || !bc_angle) {
-    set_error("NULL argument to cvchip_ctx_create");
-    return CVCHIP_ERR_NULL;
-  }
-  for (int m = 0; m < 4; m++)
-    if (dims[m] < 2) {
-      set_error("lattice extent %d in direction %d is below 2", dims[m], m);
+  return cvchip::guarded("cvchip_ctx_create", [&]() -> int {
+    if (!ctx || !dims || !bc_angle) {
+      set_error("NULL argument to cvchip_ctx_create");
+      return CVCHIP_ERR_NULL;
+    }
+    for (int m = 0; m < 4; m++)
+      if (dims[m] < 2) {
+        set_error("lattice extent %d in direction %d is below 2", dims[m], m);
+        return CVCHIP_ERR_ARG;
+      }
+    int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    if (V >= ((int64_t)1 << 31)) {
+      set_error("lattice volume %lld too large", (long long)V);
       return CVCHIP_ERR_ARG;
     }
-  int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
-  if (V >= ((int64_t)1 << 31)) {
-    set_error("lattice volume %lld too large", (long long)V);
-    return CVCHIP_ERR_ARG;
-  }
-  if (!(kappa > 0.0)) {
-    set_error("kappa must be positive");
-    return CVCHIP_ERR_ARG;
-  }
-  cvchip_ctx *c = new cvchip_ctx();
-  c->lat.T = dims[0];
-  c->lat.LX = dims[1];
-  c->lat.LY = dims[2];
-  c->lat.LZ = dims[3];
-  c->lat.V = V;
-  c->kappa = kappa;
-  c->mu = mu;
-  for (int m = 0; m < 4; m++) {
-    c->bc[m] = bc_angle[m];
-    // cvc_geometry.c:452-459: co_phase_up[mu] = exp(i pi BCangle[mu] / L_mu)
-    c->ph.re[m] = cos(bc_angle[m] * M_PI / (double)dims[m]);
-    c->ph.im[m] = sin(bc_angle[m] * M_PI / (double)dims[m]);
-  }
-  hipError_t e = hipGetDevice(&c->device);
-  if (e == hipSuccess) {
-    const int rs = twopoint_scratch_init(c->device);
-    if (rs) {
-      cvchip_ctx_destroy(c);
-      return rs;
+    if (!(kappa > 0.0)) {
+      set_error("kappa must be positive");
+      return CVCHIP_ERR_ARG;
     }
-  }
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-  if (e != hipSuccess) {
-    int rc = hip_fail(e, "context setup", __FILE__, __LINE__);
-    cvchip_ctx_destroy(c);
-    return rc;
-  }
-  *ctx = c;
-  return CVCHIP_OK;
+    cvchip_ctx *c = new cvchip_ctx();
+    c->lat.T = dims[0];
+    c->lat.LX = dims[1];
+    c->lat.LY = dims[2];
+    c->lat.LZ = dims[3];
+    c->lat.V = V;
+    c->kappa = kappa;
+    c->mu = mu;
+    for (int m = 0; m < 4; m++) {
+      c->bc[m] = bc_angle[m];
+      // cvc_geometry.c:452-459: co_phase_up[mu] = exp(i pi BCangle[mu] / L_mu)
+      c->ph.re[m] = cos(bc_angle[m] * M_PI / (double)dims[m]);
+      c->ph.im[m] = sin(bc_angle[m] * M_PI / (double)dims[m]);
+    }
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) {
+      const int rs = twopoint_scratch_init(c->device);
+      if (rs) {
+        cvchip_ctx_destroy(c);
+        return rs;
+      }
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+      int rc = hip_fail(e, "context setup", __FILE__, __LINE__);
+      cvchip_ctx_destroy(c);
+      return rc;
+    }
+    *ctx = c;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_ctx_destroy(cvchip_ctx *c) {
-  if (!c) return CVCHIP_OK;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  pool_release(c);
-  if (c->gauge64) (void)hipFree(c->gauge64);
-  if (c->gauge64r12) (void)hipFree(c->gauge64r12);
-  if (c->gauge32r12) (void)hipFree(c->gauge32r12);
-  if (c->stage) (void)hipFree(c->stage);
-  for (int k = 0; k < 2; k++) {
-    if (c->pin[k]) (void)hipHostFree(c->pin[k]);
-    if (c->pin_ev[k]) (void)hipEventDestroy(c->pin_ev[k]);
-  }
-  if (c->red.ptr) (void)hipFree(c->red.ptr);
-  if (c->red2.ptr) (void)hipFree(c->red2.ptr);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_ctx_destroy", [&]() -> int {
+    if (!c) return CVCHIP_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pool_release(c);
+    if (c->gauge64) (void)hipFree(c->gauge64);
+    if (c->gauge64r12) (void)hipFree(c->gauge64r12);
+    if (c->gauge32r12) (void)hipFree(c->gauge32r12);
+    if (c->stage) (void)hipFree(c->stage);
+    for (int k = 0; k < 2; k++) {
+      if (c->pin[k]) (void)hipHostFree(c->pin[k]);
+      if (c->pin_ev[k]) (void)hipEventDestroy(c->pin_ev[k]);
+    }
+    if (c->red.ptr) (void)hipFree(c->red.ptr);
+    if (c->red2.ptr) (void)hipFree(c->red2.ptr);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_ctx_set_params(cvchip_ctx *c, double kappa, double mu) {
-  if (!c) return CVCHIP_ERR_NULL;
-  if (!(kappa > 0.0)) {
-    set_error("kappa must be positive");
-    return CVCHIP_ERR_ARG;
-  }
-  c->kappa = kappa;
-  c->mu = mu;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_ctx_set_params", [&]() -> int {
+    if (!c) return CVCHIP_ERR_NULL;
+    if (!(kappa > 0.0)) {
+      set_error("kappa must be positive");
+      return CVCHIP_ERR_ARG;
+    }
+    c->kappa = kappa;
+    c->mu = mu;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_ctx_set_link_recon(cvchip_ctx *c, int recon) {
-  if (!c) return CVCHIP_ERR_NULL;
-  if (recon != 18 && recon != 12) {
-    set_error("link reconstruction must be 18 or 12, got %d", recon);
-    return CVCHIP_ERR_ARG;
-  }
-  c->link_recon = recon;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_ctx_set_link_recon", [&]() -> int {
+    if (!c) return CVCHIP_ERR_NULL;
+    if (recon != 18 && recon != 12) {
+      set_error("link reconstruction must be 18 or 12, got %d", recon);
+      return CVCHIP_ERR_ARG;
+    }
+    c->link_recon = recon;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_ctx_set_solver_flags(cvchip_ctx *c, unsigned flags) {
-  if (!c) return CVCHIP_ERR_NULL;
-  if (flags & ~(CVCHIP_SOLVE_LAGGED | CVCHIP_SOLVE_VERBOSE | CVCHIP_SOLVE_EO)) {
-    set_error("unknown solver flags 0x%x", flags);
-    return CVCHIP_ERR_ARG;
-  }
-  if ((flags & CVCHIP_SOLVE_EO) && c->lat.LZ % 2) {
-    set_error("even/odd preconditioning needs an even LZ (got %d)", c->lat.LZ);
-    return CVCHIP_ERR_ARG;
-  }
-  c->solve_flags = flags;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_ctx_set_solver_flags", [&]() -> int {
+    if (!c) return CVCHIP_ERR_NULL;
+    if (flags & ~(CVCHIP_SOLVE_LAGGED | CVCHIP_SOLVE_VERBOSE | CVCHIP_SOLVE_EO)) {
+      set_error("unknown solver flags 0x%x", flags);
+      return CVCHIP_ERR_ARG;
+    }
+    if ((flags & CVCHIP_SOLVE_EO) && c->lat.LZ % 2) {
+      set_error("even/odd preconditioning needs an even LZ (got %d)", c->lat.LZ);
+      return CVCHIP_ERR_ARG;
+    }
+    c->solve_flags = flags;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_ctx_set_dslash_policy(cvchip_ctx *c, int policy) {
-  if (!c) return CVCHIP_ERR_NULL;
-  const int mode = (policy >> 4) & 15, yb = (policy >> 8) & 255;
-  const int bt = (policy >> 16) & 15, bx = (policy >> 20) & 15;
-  if (policy < 0 || policy >= (1 << 28) || mode > 6) {
-    set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..6, 8-15 y-block "
-              "(traversal 6: march length), 16-19 brick t (traversal 5: log2 of the column's z extent; 6: patch x), "
-              "20-23 brick x (traversal 6: patch y), 24-27 LDS padding",
-              policy);
-    return CVCHIP_ERR_ARG;
-  }
-  if (mode == 6) {
-    // t march: patch extents divide LX and LY
-    if ((bt && c->lat.LX % bt) || (bx && c->lat.LY % bx)) {
-      set_error("operator policy 0x%x: t-march patch extents x %d, y %d must divide LX=%d, LY=%d", policy, bt, bx,
-                c->lat.LX, c->lat.LY);
+  return cvchip::guarded("cvchip_ctx_set_dslash_policy", [&]() -> int {
+    if (!c) return CVCHIP_ERR_NULL;
+    const int mode = (policy >> 4) & 15, yb = (policy >> 8) & 255;
+    const int bt = (policy >> 16) & 15, bx = (policy >> 20) & 15;
+    if (policy < 0 || policy >= (1 << 28) || mode > 6) {
+      set_error("operator policy 0x%x: bits 0-3 staging/streaming, 4-7 traversal 0..6, 8-15 y-block "
+                "(traversal 6: march length), 16-19 brick t (traversal 5: log2 of the column's z extent; 6: patch x), "
+                "20-23 brick x (traversal 6: patch y), 24-27 LDS padding",
+                policy);
       return CVCHIP_ERR_ARG;
     }
-  } else if (mode == 5) {
-    // column extents: powers of two dividing the lattice
-    const int zb = bt ? 1 << bt : 8;
-    const int cy = yb ? yb : 4, cx = bx ? bx : 4;
-    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
-    if (bt > 5 || !pow2(cy) || !pow2(cx) || c->lat.LY % cy || c->lat.LX % cx || c->lat.LZ % zb) {
-      set_error("operator policy 0x%x: column extents x %d, y %d, z %d must be powers of two dividing LX=%d, "
-                "LY=%d, LZ=%d", policy, cx, cy, zb, c->lat.LX, c->lat.LY, c->lat.LZ);
+    if (mode == 6) {
+      // t march: patch extents divide LX and LY
+      if ((bt && c->lat.LX % bt) || (bx && c->lat.LY % bx)) {
+        set_error("operator policy 0x%x: t-march patch extents x %d, y %d must divide LX=%d, LY=%d", policy, bt, bx,
+                  c->lat.LX, c->lat.LY);
+        return CVCHIP_ERR_ARG;
+      }
+    } else if (mode == 5) {
+      // column extents: powers of two dividing the lattice
+      const int zb = bt ? 1 << bt : 8;
+      const int cy = yb ? yb : 4, cx = bx ? bx : 4;
+      auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+      if (bt > 5 || !pow2(cy) || !pow2(cx) || c->lat.LY % cy || c->lat.LX % cx || c->lat.LZ % zb) {
+        set_error("operator policy 0x%x: column extents x %d, y %d, z %d must be powers of two dividing LX=%d, "
+                  "LY=%d, LZ=%d", policy, cx, cy, zb, c->lat.LX, c->lat.LY, c->lat.LZ);
+        return CVCHIP_ERR_ARG;
+      }
+    } else if ((yb && c->lat.LY % yb) || (bt && c->lat.T % bt) || (bx && c->lat.LX % bx)) {
+      set_error("operator policy 0x%x: block extents must divide the lattice (T=%d LX=%d LY=%d)", policy,
+                c->lat.T, c->lat.LX, c->lat.LY);
       return CVCHIP_ERR_ARG;
     }
-  } else if ((yb && c->lat.LY % yb) || (bt && c->lat.T % bt) || (bx && c->lat.LX % bx)) {
-    set_error("operator policy 0x%x: block extents must divide the lattice (T=%d LX=%d LY=%d)", policy,
-              c->lat.T, c->lat.LX, c->lat.LY);
-    return CVCHIP_ERR_ARG;
-  }
-  c->dslash_policy = policy;
-  return CVCHIP_OK;
+    c->dslash_policy = policy;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_set_gauge(cvchip_ctx *c, const double *gauge) {
-  if (!c || !gauge) {
-    set_error("NULL argument to cvchip_set_gauge");
-    return CVCHIP_ERR_NULL;
-  }
-  const int64_t V = c->lat.V;
-  if (!c->gauge64) CVC_HIP(hipMalloc(&c->gauge64, sizeof(double2) * 36 * V));
-  if (!c->gauge64r12) CVC_HIP(hipMalloc(&c->gauge64r12, sizeof(double2) * 24 * V));
-  if (!c->gauge32r12) CVC_HIP(hipMalloc(&c->gauge32r12, sizeof(float2) * 24 * V));
-  CVC_TRY(ensure_stage(c, sizeof(double) * 72 * V));
-  CVC_TRY(copy_to_device(c, c->stage, gauge, sizeof(double) * 72 * V));
-  const int64_t n = 36 * V;
-  hipLaunchKernelGGL(k_gauge_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     c->gauge64, c->gauge64r12, c->gauge32r12, (const double *)c->stage, V);
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  c->gauge_set = 1;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_set_gauge", [&]() -> int {
+    if (!c || !gauge) {
+      set_error("NULL argument to cvchip_set_gauge");
+      return CVCHIP_ERR_NULL;
+    }
+    const int64_t V = c->lat.V;
+    if (!c->gauge64) CVC_HIP(hipMalloc(&c->gauge64, sizeof(double2) * 36 * V));
+    if (!c->gauge64r12) CVC_HIP(hipMalloc(&c->gauge64r12, sizeof(double2) * 24 * V));
+    if (!c->gauge32r12) CVC_HIP(hipMalloc(&c->gauge32r12, sizeof(float2) * 24 * V));
+    CVC_TRY(ensure_stage(c, sizeof(double) * 72 * V));
+    CVC_TRY(copy_to_device(c, c->stage, gauge, sizeof(double) * 72 * V));
+    const int64_t n = 36 * V;
+    hipLaunchKernelGGL(k_gauge_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->gauge64, c->gauge64r12, c->gauge32r12, (const double *)c->stage, V);
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    c->gauge_set = 1;
+    return CVCHIP_OK;
+  });
 }
 
 namespace cvchip {
@@ -675,31 +697,39 @@ int download_aos(cvchip_ctx *c, const Field &f, double *host, int rhs0, int n) {
 }  // namespace cvchip
 
 extern "C" int cvchip_field_create(cvchip_ctx *c, int nrhs, int prec, cvchip_field **out) {
-  if (!c || !out) return CVCHIP_ERR_NULL;
-  cvchip_field *f = new cvchip_field();
-  int rc = field_alloc(c, f->f, nrhs, prec, 0, false);
-  if (rc) {
-    field_free(f->f);
-    delete f;
-    return rc;
-  }
-  *out = f;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_create", [&]() -> int {
+    if (!c || !out) return CVCHIP_ERR_NULL;
+    cvchip_field *f = new cvchip_field();
+    int rc = field_alloc(c, f->f, nrhs, prec, 0, false);
+    if (rc) {
+      field_free(f->f);
+      delete f;
+      return rc;
+    }
+    *out = f;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_destroy(cvchip_field *f) {
-  if (!f) return CVCHIP_OK;
-  field_free(f->f);
-  delete f;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_destroy", [&]() -> int {
+    if (!f) return CVCHIP_OK;
+    field_free(f->f);
+    delete f;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_upload(cvchip_ctx *c, cvchip_field *f, const double *host, int rhs0, int n) {
-  if (!c || !f || !host) return CVCHIP_ERR_NULL;
-  return upload_aos(c, f->f, host, rhs0, n);
+  return cvchip::guarded("cvchip_field_upload", [&]() -> int {
+    if (!c || !f || !host) return CVCHIP_ERR_NULL;
+    return upload_aos(c, f->f, host, rhs0, n);
+  });
 }
 
 extern "C" int cvchip_field_download(cvchip_ctx *c, const cvchip_field *f, double *host, int rhs0, int n) {
-  if (!c || !f || !host) return CVCHIP_ERR_NULL;
-  return download_aos(c, f->f, host, rhs0, n);
+  return cvchip::guarded("cvchip_field_download", [&]() -> int {
+    if (!c || !f || !host) return CVCHIP_ERR_NULL;
+    return download_aos(c, f->f, host, rhs0, n);
+  });
 }

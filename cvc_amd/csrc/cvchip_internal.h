@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/cvchip.h"
+#include "entry_guard.h"
 
 namespace cvchip {
 

@@ -1367,41 +1367,47 @@ using namespace cvchip;
 
 extern "C" int cvchip_field_dslash(cvchip_ctx *c, cvchip_field *out, const cvchip_field *in,
                                    double mutm, int gamma5_out) {
-  if (!c || !out || !in) return CVCHIP_ERR_NULL;
-  CVC_TRY(dslash(c, out->f, in->f, mutm, gamma5_out));
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_dslash", [&]() -> int {
+    if (!c || !out || !in) return CVCHIP_ERR_NULL;
+    CVC_TRY(dslash(c, out->f, in->f, mutm, gamma5_out));
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_dslash_timed(cvchip_ctx *c, cvchip_field *out, const cvchip_field *in,
                                          double mutm, int reps, double *ms) {
-  if (!c || !out || !in || !ms) return CVCHIP_ERR_NULL;
-  if (reps < 1) {
-    set_error("reps must be positive");
-    return CVCHIP_ERR_ARG;
-  }
-  CVC_HIP(hipEventRecord(c->ev0, c->stream));
-  for (int k = 0; k < reps; k++) CVC_TRY(dslash(c, out->f, in->f, mutm, 0));
-  CVC_HIP(hipEventRecord(c->ev1, c->stream));
-  CVC_HIP(hipEventSynchronize(c->ev1));
-  float t = 0.f;
-  CVC_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
-  *ms = (double)t / reps;
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_dslash_timed", [&]() -> int {
+    if (!c || !out || !in || !ms) return CVCHIP_ERR_NULL;
+    if (reps < 1) {
+      set_error("reps must be positive");
+      return CVCHIP_ERR_ARG;
+    }
+    CVC_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int k = 0; k < reps; k++) CVC_TRY(dslash(c, out->f, in->f, mutm, 0));
+    CVC_HIP(hipEventRecord(c->ev1, c->stream));
+    CVC_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    CVC_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = (double)t / reps;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_Q_phi(cvchip_ctx *c, double *xi, const double *phi, double mutm, int gamma5_out) {
-  if (!c || !xi || !phi) {
-    set_error("NULL argument to cvchip_Q_phi");
-    return CVCHIP_ERR_NULL;
-  }
-  Field a, b;
-  int rc = field_alloc(c, a, 1, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, b, 1, CVCHIP_PREC_F64);
-  if (!rc) rc = upload_aos(c, a, phi, 0, 1);
-  if (!rc) rc = dslash(c, b, a, mutm, gamma5_out);
-  if (!rc) rc = download_aos(c, b, xi, 0, 1);
-  field_free(a);
-  field_free(b);
-  return rc;
+  return cvchip::guarded("cvchip_Q_phi", [&]() -> int {
+    if (!c || !xi || !phi) {
+      set_error("NULL argument to cvchip_Q_phi");
+      return CVCHIP_ERR_NULL;
+    }
+    Field a, b;
+    int rc = field_alloc(c, a, 1, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, b, 1, CVCHIP_PREC_F64);
+    if (!rc) rc = upload_aos(c, a, phi, 0, 1);
+    if (!rc) rc = dslash(c, b, a, mutm, gamma5_out);
+    if (!rc) rc = download_aos(c, b, xi, 0, 1);
+    field_free(a);
+    field_free(b);
+    return rc;
+  });
 }

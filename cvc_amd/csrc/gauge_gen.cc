@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include "../../include/cvchip.h"
+#include "entry_guard.h"
 
 namespace {
 
@@ -129,12 +130,14 @@ void one_link(uint64_t seed, int64_t site, int mu, double eps, double *out18) {
 }  // namespace
 
 extern "C" int cvchip_gauge_random(const int dims[4], uint64_t seed, double eps, double *gauge) {
-  if (!dims || !gauge) return CVCHIP_ERR_NULL;
-  for (int m = 0; m < 4; m++)
-    if (dims[m] <= 0) return CVCHIP_ERR_ARG;
-  const int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
-#pragma omp parallel for schedule(static)
-  for (int64_t s = 0; s < V; s++)
-    for (int mu = 0; mu < 4; mu++) one_link(seed, s, mu, eps, gauge + 72 * s + 18 * mu);
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_gauge_random", [&]() -> int {
+    if (!dims || !gauge) return CVCHIP_ERR_NULL;
+    for (int m = 0; m < 4; m++)
+      if (dims[m] <= 0) return CVCHIP_ERR_ARG;
+    const int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  #pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < V; s++)
+      for (int mu = 0; mu < 4; mu++) one_link(seed, s, mu, eps, gauge + 72 * s + 18 * mu);
+    return CVCHIP_OK;
+  });
 }

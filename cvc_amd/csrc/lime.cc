@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/cvchip.h"
+#include "entry_guard.h"
 
 namespace cvchip {
 void set_error(const char *fmt, ...);
@@ -139,7 +140,7 @@ bool scan(FILE *f, std::vector<Record> &recs) {
     t[128] = 0;
     r.type = t;
     r.offset = (uint64_t)pos + 144;
-    if ((long long)(r.offset + r.bytes) > size) return false;
+    if (r.bytes > (uint64_t)size || r.offset + r.bytes > (uint64_t)size) return false;   // no wrap-around
     recs.push_back(r);
     pos = (long long)(r.offset + r.bytes + (8 - r.bytes % 8) % 8);
   }
@@ -188,53 +189,55 @@ bool check_dims(const int dims[4]) {
 extern "C" int cvchip_lime_write_contraction(const char *filename, const double *s, int N, int prec,
                                              const int dims[4], const char *type, int nconf, int source,
                                              uint32_t checksum[2]) {
-  if (!filename || !s || !dims || !type) {
-    set_error("NULL argument to cvchip_lime_write_contraction");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims) || N < 1 || (prec != 64 && prec != 32)) {
-    set_error("invalid contraction format (N %d, precision %d)", N, prec);
-    return CVCHIP_ERR_ARG;
-  }
-  FILE *f = fopen(filename, "wb");
-  if (!f) {
-    set_error("could not open %s for writing", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  const time_t now = time(nullptr);
-  char msg[2048];
-  snprintf(msg, sizeof(msg),
-           "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<cvcFormat>\n<type>%s</type>\n<precision>%d</precision>\n"
-           "<components>%d</components>\n<lx>%d</lx>\n<ly>%d</ly>\n<lz>%d</lz>\n<lt>%d</lt>\n<nconf>%d</nconf>\n"
-           "<source>%d</source>\n</cvcFormat>\ndate %s",
-           type, prec, N, dims[1], dims[2], dims[3], dims[0], nconf, source, ctime(&now));
-  bool ok = write_record(f, "cvc_contraction-format", true, true, msg, strlen(msg));
-  const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-  const size_t w = prec == 64 ? 8 : 4, site_bytes = (size_t)2 * N * w;
-  std::vector<unsigned char> data(V * site_bytes);
-  Checksum ck;
-  for (uint64_t ix = 0; ix < V; ix++) {
-    unsigned char *p = data.data() + ix * site_bytes;
-    for (int k = 0; k < 2 * N; k++) {
-      const double v = s[ix * 2 * N + k];
-      if (prec == 64) put_double_be(p + k * w, v);
-      else put_float_be(p + k * w, (float)v);
+  return cvchip::guarded("cvchip_lime_write_contraction", [&]() -> int {
+    if (!filename || !s || !dims || !type) {
+      set_error("NULL argument to cvchip_lime_write_contraction");
+      return CVCHIP_ERR_NULL;
     }
-    ck.accum(ix, p, site_bytes);
-  }
-  ok = ok && write_record(f, "scidac-binary-data", true, false, data.data(), data.size());
-  const std::string x = checksum_xml(ck);
-  ok = ok && write_record(f, "scidac-checksum", false, true, x.data(), x.size());
-  ok = (fclose(f) == 0) && ok;
-  if (!ok) {
-    set_error("write error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  if (checksum) {
-    checksum[0] = ck.a;
-    checksum[1] = ck.b;
-  }
-  return CVCHIP_OK;
+    if (!check_dims(dims) || N < 1 || (prec != 64 && prec != 32)) {
+      set_error("invalid contraction format (N %d, precision %d)", N, prec);
+      return CVCHIP_ERR_ARG;
+    }
+    FILE *f = fopen(filename, "wb");
+    if (!f) {
+      set_error("could not open %s for writing", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    const time_t now = time(nullptr);
+    char msg[2048];
+    snprintf(msg, sizeof(msg),
+             "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<cvcFormat>\n<type>%s</type>\n<precision>%d</precision>\n"
+             "<components>%d</components>\n<lx>%d</lx>\n<ly>%d</ly>\n<lz>%d</lz>\n<lt>%d</lt>\n<nconf>%d</nconf>\n"
+             "<source>%d</source>\n</cvcFormat>\ndate %s",
+             type, prec, N, dims[1], dims[2], dims[3], dims[0], nconf, source, ctime(&now));
+    bool ok = write_record(f, "cvc_contraction-format", true, true, msg, strlen(msg));
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    const size_t w = prec == 64 ? 8 : 4, site_bytes = (size_t)2 * N * w;
+    std::vector<unsigned char> data(V * site_bytes);
+    Checksum ck;
+    for (uint64_t ix = 0; ix < V; ix++) {
+      unsigned char *p = data.data() + ix * site_bytes;
+      for (int k = 0; k < 2 * N; k++) {
+        const double v = s[ix * 2 * N + k];
+        if (prec == 64) put_double_be(p + k * w, v);
+        else put_float_be(p + k * w, (float)v);
+      }
+      ck.accum(ix, p, site_bytes);
+    }
+    ok = ok && write_record(f, "scidac-binary-data", true, false, data.data(), data.size());
+    const std::string x = checksum_xml(ck);
+    ok = ok && write_record(f, "scidac-checksum", false, true, x.data(), x.size());
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+      set_error("write error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    if (checksum) {
+      checksum[0] = ck.a;
+      checksum[1] = ck.b;
+    }
+    return CVCHIP_OK;
+  });
 }
 
 // read_lime_contraction (reference contractions_io.c:430-505): the position-th
@@ -243,72 +246,74 @@ extern "C" int cvchip_lime_write_contraction(const char *filename, const double 
 // when there is one.
 extern "C" int cvchip_lime_read_contraction(const char *filename, double *s, int N, int position,
                                             const int dims[4], uint32_t checksum[2]) {
-  if (!filename || !s || !dims) {
-    set_error("NULL argument to cvchip_lime_read_contraction");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims) || N < 1 || position < 0) {
-    set_error("invalid contraction read request (N %d, position %d)", N, position);
-    return CVCHIP_ERR_ARG;
-  }
-  FILE *f = fopen(filename, "rb");
-  if (!f) {
-    set_error("could not open %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<Record> recs;
-  if (!scan(f, recs)) {
-    fclose(f);
-    set_error("%s is not a LIME file", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  int got = -1;
-  size_t k = 0;
-  for (; k < recs.size(); k++)
-    if (recs[k].type == "scidac-binary-data" && ++got == position) break;
-  if (k == recs.size()) {
-    fclose(f);
-    set_error("no scidac-binary-data record %d in %s", position, filename);
-    return CVCHIP_ERR_ARG;
-  }
-  const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-  int prec;
-  if (recs[k].bytes == V * 2 * N * 8) prec = 64;
-  else if (recs[k].bytes == V * 2 * N * 4) prec = 32;
-  else {
-    fclose(f);
-    set_error("wrong length %llu of the contraction record in %s", (unsigned long long)recs[k].bytes, filename);
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<unsigned char> data;
-  if (!read_payload(f, recs[k], data)) {
-    fclose(f);
-    set_error("read error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  const size_t w = prec == 64 ? 8 : 4, site_bytes = (size_t)2 * N * w;
-  Checksum ck;
-  for (uint64_t ix = 0; ix < V; ix++) {
-    const unsigned char *p = data.data() + ix * site_bytes;
-    ck.accum(ix, p, site_bytes);
-    for (int j = 0; j < 2 * N; j++)
-      s[ix * 2 * N + j] = prec == 64 ? get_double_be(p + j * w) : (double)get_float_be(p + j * w);
-  }
-  int rc = CVCHIP_OK;
-  if (k + 1 < recs.size() && recs[k + 1].type == "scidac-checksum") {
-    std::vector<unsigned char> xml;
-    uint32_t a, b;
-    if (read_payload(f, recs[k + 1], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
-      set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
-      rc = CVCHIP_ERR_ARG;
+  return cvchip::guarded("cvchip_lime_read_contraction", [&]() -> int {
+    if (!filename || !s || !dims) {
+      set_error("NULL argument to cvchip_lime_read_contraction");
+      return CVCHIP_ERR_NULL;
     }
-  }
-  fclose(f);
-  if (checksum) {
-    checksum[0] = ck.a;
-    checksum[1] = ck.b;
-  }
-  return rc;
+    if (!check_dims(dims) || N < 1 || position < 0) {
+      set_error("invalid contraction read request (N %d, position %d)", N, position);
+      return CVCHIP_ERR_ARG;
+    }
+    FILE *f = fopen(filename, "rb");
+    if (!f) {
+      set_error("could not open %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<Record> recs;
+    if (!scan(f, recs)) {
+      fclose(f);
+      set_error("%s is not a LIME file", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    int got = -1;
+    size_t k = 0;
+    for (; k < recs.size(); k++)
+      if (recs[k].type == "scidac-binary-data" && ++got == position) break;
+    if (k == recs.size()) {
+      fclose(f);
+      set_error("no scidac-binary-data record %d in %s", position, filename);
+      return CVCHIP_ERR_ARG;
+    }
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    int prec;
+    if (recs[k].bytes == V * 2 * N * 8) prec = 64;
+    else if (recs[k].bytes == V * 2 * N * 4) prec = 32;
+    else {
+      fclose(f);
+      set_error("wrong length %llu of the contraction record in %s", (unsigned long long)recs[k].bytes, filename);
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<unsigned char> data;
+    if (!read_payload(f, recs[k], data)) {
+      fclose(f);
+      set_error("read error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    const size_t w = prec == 64 ? 8 : 4, site_bytes = (size_t)2 * N * w;
+    Checksum ck;
+    for (uint64_t ix = 0; ix < V; ix++) {
+      const unsigned char *p = data.data() + ix * site_bytes;
+      ck.accum(ix, p, site_bytes);
+      for (int j = 0; j < 2 * N; j++)
+        s[ix * 2 * N + j] = prec == 64 ? get_double_be(p + j * w) : (double)get_float_be(p + j * w);
+    }
+    int rc = CVCHIP_OK;
+    if (k + 1 < recs.size() && recs[k + 1].type == "scidac-checksum") {
+      std::vector<unsigned char> xml;
+      uint32_t a, b;
+      if (read_payload(f, recs[k + 1], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
+        set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
+        rc = CVCHIP_ERR_ARG;
+      }
+    }
+    fclose(f);
+    if (checksum) {
+      checksum[0] = ck.a;
+      checksum[1] = ck.b;
+    }
+    return rc;
+  });
 }
 
 // ------------------------------------------------------------ gauge (ILDG)
@@ -320,145 +325,149 @@ extern "C" int cvchip_lime_read_contraction(const char *filename, double *s, int
 // z with z fastest).  The checksum rank is the file order.
 extern "C" int cvchip_lime_read_gauge(const char *filename, const int dims[4], double *gauge,
                                       uint32_t checksum[2]) {
-  if (!filename || !dims || !gauge) {
-    set_error("NULL argument to cvchip_lime_read_gauge");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims)) {
-    set_error("invalid lattice extents");
-    return CVCHIP_ERR_ARG;
-  }
-  FILE *f = fopen(filename, "rb");
-  if (!f) {
-    set_error("could not open %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<Record> recs;
-  if (!scan(f, recs)) {
-    fclose(f);
-    set_error("%s is not a LIME file", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  size_t k = 0;
-  while (k < recs.size() && recs[k].type != "ildg-binary-data") k++;
-  if (k == recs.size()) {
-    fclose(f);
-    set_error("no ildg-binary-data record in %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  const int T = dims[0], LX = dims[1], LY = dims[2], LZ = dims[3];
-  const uint64_t V = (uint64_t)T * LX * LY * LZ;
-  int w;
-  if (recs[k].bytes == V * 72 * 8) w = 8;
-  else if (recs[k].bytes == V * 72 * 4) w = 4;
-  else {
-    fclose(f);
-    set_error("wrong lattice size or precision in %s (%llu bytes)", filename, (unsigned long long)recs[k].bytes);
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<unsigned char> data;
-  if (!read_payload(f, recs[k], data)) {
-    fclose(f);
-    set_error("read error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  Checksum ck;
-  uint64_t rank = 0;
-  for (int t = 0; t < T; t++)
-    for (int z = 0; z < LZ; z++)
-      for (int y = 0; y < LY; y++)
-        for (int x = 0; x < LX; x++, rank++) {
-          const unsigned char *p = data.data() + rank * 72 * w;
-          ck.accum(rank, p, (size_t)72 * w);
-          const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
-          for (int m = 0; m < 4; m++) {
-            const int mu = m == 3 ? 0 : m + 1;   // file order x, y, z, t
-            for (int e = 0; e < 18; e++) {
-              const unsigned char *q = p + (size_t)(m * 18 + e) * w;
-              gauge[(ix * 4 + mu) * 18 + e] = w == 8 ? get_double_be(q) : (double)get_float_be(q);
+  return cvchip::guarded("cvchip_lime_read_gauge", [&]() -> int {
+    if (!filename || !dims || !gauge) {
+      set_error("NULL argument to cvchip_lime_read_gauge");
+      return CVCHIP_ERR_NULL;
+    }
+    if (!check_dims(dims)) {
+      set_error("invalid lattice extents");
+      return CVCHIP_ERR_ARG;
+    }
+    FILE *f = fopen(filename, "rb");
+    if (!f) {
+      set_error("could not open %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<Record> recs;
+    if (!scan(f, recs)) {
+      fclose(f);
+      set_error("%s is not a LIME file", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    size_t k = 0;
+    while (k < recs.size() && recs[k].type != "ildg-binary-data") k++;
+    if (k == recs.size()) {
+      fclose(f);
+      set_error("no ildg-binary-data record in %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    const int T = dims[0], LX = dims[1], LY = dims[2], LZ = dims[3];
+    const uint64_t V = (uint64_t)T * LX * LY * LZ;
+    int w;
+    if (recs[k].bytes == V * 72 * 8) w = 8;
+    else if (recs[k].bytes == V * 72 * 4) w = 4;
+    else {
+      fclose(f);
+      set_error("wrong lattice size or precision in %s (%llu bytes)", filename, (unsigned long long)recs[k].bytes);
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<unsigned char> data;
+    if (!read_payload(f, recs[k], data)) {
+      fclose(f);
+      set_error("read error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    Checksum ck;
+    uint64_t rank = 0;
+    for (int t = 0; t < T; t++)
+      for (int z = 0; z < LZ; z++)
+        for (int y = 0; y < LY; y++)
+          for (int x = 0; x < LX; x++, rank++) {
+            const unsigned char *p = data.data() + rank * 72 * w;
+            ck.accum(rank, p, (size_t)72 * w);
+            const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
+            for (int m = 0; m < 4; m++) {
+              const int mu = m == 3 ? 0 : m + 1;   // file order x, y, z, t
+              for (int e = 0; e < 18; e++) {
+                const unsigned char *q = p + (size_t)(m * 18 + e) * w;
+                gauge[(ix * 4 + mu) * 18 + e] = w == 8 ? get_double_be(q) : (double)get_float_be(q);
+              }
             }
           }
+    int rc = CVCHIP_OK;
+    for (size_t j = k + 1; j < recs.size(); j++)
+      if (recs[j].type == "scidac-checksum") {
+        std::vector<unsigned char> xml;
+        uint32_t a, b;
+        if (read_payload(f, recs[j], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
+          set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
+          rc = CVCHIP_ERR_ARG;
         }
-  int rc = CVCHIP_OK;
-  for (size_t j = k + 1; j < recs.size(); j++)
-    if (recs[j].type == "scidac-checksum") {
-      std::vector<unsigned char> xml;
-      uint32_t a, b;
-      if (read_payload(f, recs[j], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
-        set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
-        rc = CVCHIP_ERR_ARG;
+        break;
       }
-      break;
+    fclose(f);
+    if (checksum) {
+      checksum[0] = ck.a;
+      checksum[1] = ck.b;
     }
-  fclose(f);
-  if (checksum) {
-    checksum[0] = ck.a;
-    checksum[1] = ck.b;
-  }
-  return rc;
+    return rc;
+  });
 }
 
 // An ILDG gauge file the reader above (and the reference's) accepts:
 // "ildg-format" XML, "ildg-binary-data" (64 or 32 bit), "scidac-checksum".
 extern "C" int cvchip_lime_write_gauge(const char *filename, const int dims[4], const double *gauge, int prec,
                                        uint32_t checksum[2]) {
-  if (!filename || !dims || !gauge) {
-    set_error("NULL argument to cvchip_lime_write_gauge");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims) || (prec != 64 && prec != 32)) {
-    set_error("invalid gauge format (precision %d)", prec);
-    return CVCHIP_ERR_ARG;
-  }
-  FILE *f = fopen(filename, "wb");
-  if (!f) {
-    set_error("could not open %s for writing", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  const int T = dims[0], LX = dims[1], LY = dims[2], LZ = dims[3];
-  char msg[1024];
-  snprintf(msg, sizeof(msg),
-           "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<ildgFormat xmlns=\"http://www.lqcd.org/ildg\"\n"
-           "            xmlns:xsi=\"http://www.w3.org/2001/XMLSchema-instance\"\n"
-           "            xsi:schemaLocation=\"http://www.lqcd.org/ildg http://www.lqcd.org/ildg/filefmt.xsd\">\n"
-           "  <version>1.0</version>\n  <field>su3gauge</field>\n  <precision>%d</precision>\n"
-           "  <lx>%d</lx>\n  <ly>%d</ly>\n  <lz>%d</lz>\n  <lt>%d</lt>\n</ildgFormat>",
-           prec, LX, LY, LZ, T);
-  bool ok = write_record(f, "ildg-format", true, false, msg, strlen(msg));
-  const uint64_t V = (uint64_t)T * LX * LY * LZ;
-  const size_t w = prec == 64 ? 8 : 4;
-  std::vector<unsigned char> data(V * 72 * w);
-  Checksum ck;
-  uint64_t rank = 0;
-  for (int t = 0; t < T; t++)
-    for (int z = 0; z < LZ; z++)
-      for (int y = 0; y < LY; y++)
-        for (int x = 0; x < LX; x++, rank++) {
-          unsigned char *p = data.data() + rank * 72 * w;
-          const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
-          for (int m = 0; m < 4; m++) {
-            const int mu = m == 3 ? 0 : m + 1;
-            for (int e = 0; e < 18; e++) {
-              const double v = gauge[(ix * 4 + mu) * 18 + e];
-              if (w == 8) put_double_be(p + (size_t)(m * 18 + e) * w, v);
-              else put_float_be(p + (size_t)(m * 18 + e) * w, (float)v);
+  return cvchip::guarded("cvchip_lime_write_gauge", [&]() -> int {
+    if (!filename || !dims || !gauge) {
+      set_error("NULL argument to cvchip_lime_write_gauge");
+      return CVCHIP_ERR_NULL;
+    }
+    if (!check_dims(dims) || (prec != 64 && prec != 32)) {
+      set_error("invalid gauge format (precision %d)", prec);
+      return CVCHIP_ERR_ARG;
+    }
+    FILE *f = fopen(filename, "wb");
+    if (!f) {
+      set_error("could not open %s for writing", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    const int T = dims[0], LX = dims[1], LY = dims[2], LZ = dims[3];
+    char msg[1024];
+    snprintf(msg, sizeof(msg),
+             "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<ildgFormat xmlns=\"http://www.lqcd.org/ildg\"\n"
+             "            xmlns:xsi=\"http://www.w3.org/2001/XMLSchema-instance\"\n"
+             "            xsi:schemaLocation=\"http://www.lqcd.org/ildg http://www.lqcd.org/ildg/filefmt.xsd\">\n"
+             "  <version>1.0</version>\n  <field>su3gauge</field>\n  <precision>%d</precision>\n"
+             "  <lx>%d</lx>\n  <ly>%d</ly>\n  <lz>%d</lz>\n  <lt>%d</lt>\n</ildgFormat>",
+             prec, LX, LY, LZ, T);
+    bool ok = write_record(f, "ildg-format", true, false, msg, strlen(msg));
+    const uint64_t V = (uint64_t)T * LX * LY * LZ;
+    const size_t w = prec == 64 ? 8 : 4;
+    std::vector<unsigned char> data(V * 72 * w);
+    Checksum ck;
+    uint64_t rank = 0;
+    for (int t = 0; t < T; t++)
+      for (int z = 0; z < LZ; z++)
+        for (int y = 0; y < LY; y++)
+          for (int x = 0; x < LX; x++, rank++) {
+            unsigned char *p = data.data() + rank * 72 * w;
+            const uint64_t ix = (((uint64_t)t * LX + x) * LY + y) * LZ + z;
+            for (int m = 0; m < 4; m++) {
+              const int mu = m == 3 ? 0 : m + 1;
+              for (int e = 0; e < 18; e++) {
+                const double v = gauge[(ix * 4 + mu) * 18 + e];
+                if (w == 8) put_double_be(p + (size_t)(m * 18 + e) * w, v);
+                else put_float_be(p + (size_t)(m * 18 + e) * w, (float)v);
+              }
             }
+            ck.accum(rank, p, (size_t)72 * w);
           }
-          ck.accum(rank, p, (size_t)72 * w);
-        }
-  ok = ok && write_record(f, "ildg-binary-data", false, false, data.data(), data.size());
-  const std::string x = checksum_xml(ck);
-  ok = ok && write_record(f, "scidac-checksum", false, true, x.data(), x.size());
-  ok = (fclose(f) == 0) && ok;
-  if (!ok) {
-    set_error("write error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  if (checksum) {
-    checksum[0] = ck.a;
-    checksum[1] = ck.b;
-  }
-  return CVCHIP_OK;
+    ok = ok && write_record(f, "ildg-binary-data", false, false, data.data(), data.size());
+    const std::string x = checksum_xml(ck);
+    ok = ok && write_record(f, "scidac-checksum", false, true, x.data(), x.size());
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+      set_error("write error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    if (checksum) {
+      checksum[0] = ck.a;
+      checksum[1] = ck.b;
+    }
+    return CVCHIP_OK;
+  });
 }
 
 // ------------------------------------------------------------ spinors (propagators)
@@ -546,66 +555,68 @@ std::string spinor_checksum_xml(const Checksum &c) {
 // scidac-checksum record that follows the data record.
 extern "C" int cvchip_lime_read_spinor(const char *filename, const int dims[4], int position, double kappa, double *s,
                                        uint32_t checksum[2]) {
-  if (!filename || !dims || !s) {
-    set_error("NULL argument to cvchip_lime_read_spinor");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims) || position < 0) {
-    set_error("invalid spinor read request (position %d)", position);
-    return CVCHIP_ERR_ARG;
-  }
-  FILE *f = fopen(filename, "rb");
-  if (!f) {
-    set_error("could not open %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<Record> recs;
-  size_t k = 0;
-  const bool lime = scan(f, recs);
-  if (lime) {
-    int got = -1;
-    for (; k < recs.size(); k++)
-      if (recs[k].type == "scidac-binary-data" && ++got == position) break;
-  }
-  if (!lime || k == recs.size()) {
-    const int rc = read_cmi(f, filename, dims, kappa, s);
-    fclose(f);
-    if (rc == CVCHIP_OK && checksum) checksum[0] = checksum[1] = 0;
-    return rc;
-  }
-  const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-  int w;
-  if (recs[k].bytes == V * 24 * 8) w = 8;
-  else if (recs[k].bytes == V * 24 * 4) w = 4;
-  else {
-    fclose(f);
-    set_error("wrong length %llu of the spinor record in %s (expected %llu)", (unsigned long long)recs[k].bytes,
-              filename, (unsigned long long)(V * 24 * 8));
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<unsigned char> data;
-  if (!read_payload(f, recs[k], data)) {
-    fclose(f);
-    set_error("read error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  Checksum ck;
-  spinor_from_file(s, dims, w, data, ck);
-  int rc = CVCHIP_OK;
-  if (k + 1 < recs.size() && recs[k + 1].type == "scidac-checksum") {
-    std::vector<unsigned char> xml;
-    uint32_t a, b;
-    if (read_payload(f, recs[k + 1], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
-      set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
-      rc = CVCHIP_ERR_ARG;
+  return cvchip::guarded("cvchip_lime_read_spinor", [&]() -> int {
+    if (!filename || !dims || !s) {
+      set_error("NULL argument to cvchip_lime_read_spinor");
+      return CVCHIP_ERR_NULL;
     }
-  }
-  fclose(f);
-  if (checksum) {
-    checksum[0] = ck.a;
-    checksum[1] = ck.b;
-  }
-  return rc;
+    if (!check_dims(dims) || position < 0) {
+      set_error("invalid spinor read request (position %d)", position);
+      return CVCHIP_ERR_ARG;
+    }
+    FILE *f = fopen(filename, "rb");
+    if (!f) {
+      set_error("could not open %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<Record> recs;
+    size_t k = 0;
+    const bool lime = scan(f, recs);
+    if (lime) {
+      int got = -1;
+      for (; k < recs.size(); k++)
+        if (recs[k].type == "scidac-binary-data" && ++got == position) break;
+    }
+    if (!lime || k == recs.size()) {
+      const int rc = read_cmi(f, filename, dims, kappa, s);
+      fclose(f);
+      if (rc == CVCHIP_OK && checksum) checksum[0] = checksum[1] = 0;
+      return rc;
+    }
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    int w;
+    if (recs[k].bytes == V * 24 * 8) w = 8;
+    else if (recs[k].bytes == V * 24 * 4) w = 4;
+    else {
+      fclose(f);
+      set_error("wrong length %llu of the spinor record in %s (expected %llu)", (unsigned long long)recs[k].bytes,
+                filename, (unsigned long long)(V * 24 * 8));
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<unsigned char> data;
+    if (!read_payload(f, recs[k], data)) {
+      fclose(f);
+      set_error("read error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    Checksum ck;
+    spinor_from_file(s, dims, w, data, ck);
+    int rc = CVCHIP_OK;
+    if (k + 1 < recs.size() && recs[k + 1].type == "scidac-checksum") {
+      std::vector<unsigned char> xml;
+      uint32_t a, b;
+      if (read_payload(f, recs[k + 1], xml) && parse_checksum(xml, a, b) && (a != ck.a || b != ck.b)) {
+        set_error("checksum mismatch in %s: file %08x %08x, data %08x %08x", filename, a, b, ck.a, ck.b);
+        rc = CVCHIP_ERR_ARG;
+      }
+    }
+    fclose(f);
+    if (checksum) {
+      checksum[0] = ck.a;
+      checksum[1] = ck.b;
+    }
+    return rc;
+  });
 }
 
 // write_lime_spinor (reference propagator_io.c:582-647): a "scidac-binary-data"
@@ -613,35 +624,37 @@ extern "C" int cvchip_lime_read_spinor(const char *filename, const int dims[4], 
 // or in a new one.
 extern "C" int cvchip_lime_write_spinor(const char *filename, const int dims[4], const double *s, int append, int prec,
                                         uint32_t checksum[2]) {
-  if (!filename || !dims || !s) {
-    set_error("NULL argument to cvchip_lime_write_spinor");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims) || (prec != 64 && prec != 32)) {
-    set_error("invalid spinor format (precision %d)", prec);
-    return CVCHIP_ERR_ARG;
-  }
-  std::vector<unsigned char> data;
-  Checksum ck;
-  spinor_to_file(s, dims, prec == 64 ? 8 : 4, data, ck);
-  FILE *f = fopen(filename, append ? "ab" : "wb");
-  if (!f) {
-    set_error("could not open %s for writing", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  bool ok = write_record(f, "scidac-binary-data", false, true, data.data(), data.size());
-  const std::string x = spinor_checksum_xml(ck);
-  ok = ok && write_record(f, "scidac-checksum", true, false, x.data(), x.size());
-  ok = (fclose(f) == 0) && ok;
-  if (!ok) {
-    set_error("write error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  if (checksum) {
-    checksum[0] = ck.a;
-    checksum[1] = ck.b;
-  }
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_lime_write_spinor", [&]() -> int {
+    if (!filename || !dims || !s) {
+      set_error("NULL argument to cvchip_lime_write_spinor");
+      return CVCHIP_ERR_NULL;
+    }
+    if (!check_dims(dims) || (prec != 64 && prec != 32)) {
+      set_error("invalid spinor format (precision %d)", prec);
+      return CVCHIP_ERR_ARG;
+    }
+    std::vector<unsigned char> data;
+    Checksum ck;
+    spinor_to_file(s, dims, prec == 64 ? 8 : 4, data, ck);
+    FILE *f = fopen(filename, append ? "ab" : "wb");
+    if (!f) {
+      set_error("could not open %s for writing", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    bool ok = write_record(f, "scidac-binary-data", false, true, data.data(), data.size());
+    const std::string x = spinor_checksum_xml(ck);
+    ok = ok && write_record(f, "scidac-checksum", true, false, x.data(), x.size());
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+      set_error("write error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    if (checksum) {
+      checksum[0] = ck.a;
+      checksum[1] = ck.b;
+    }
+    return CVCHIP_OK;
+  });
 }
 
 // write_propagator (reference propagator_io.c:125-133, the LIME build): a new
@@ -650,29 +663,31 @@ extern "C" int cvchip_lime_write_spinor(const char *filename, const int dims[4],
 // A second flavour is appended with cvchip_lime_write_spinor(append = 1).
 extern "C" int cvchip_lime_write_propagator(const char *filename, const int dims[4], const double *s, int prec,
                                             uint32_t checksum[2]) {
-  if (!filename || !dims || !s) {
-    set_error("NULL argument to cvchip_lime_write_propagator");
-    return CVCHIP_ERR_NULL;
-  }
-  if (!check_dims(dims) || (prec != 64 && prec != 32)) {
-    set_error("invalid spinor format (precision %d)", prec);
-    return CVCHIP_ERR_ARG;
-  }
-  FILE *f = fopen(filename, "wb");
-  if (!f) {
-    set_error("could not open %s for writing", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  char msg[512];
-  snprintf(msg, sizeof(msg),
-           "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<etmcFormat>\n<field>diracFermion</field>\n<precision>%d"
-           "</precision>\n<flavours>%d</flavours>\n<lx>%d</lx>\n<ly>%d</ly>\n<lz>%d</lz>\n<lt>%d</lt>\n</etmcFormat>",
-           prec, 1, dims[1], dims[2], dims[3], dims[0]);
-  bool ok = write_record(f, "etmc-propagator-format", true, false, msg, strlen(msg));
-  ok = (fclose(f) == 0) && ok;
-  if (!ok) {
-    set_error("write error on %s", filename);
-    return CVCHIP_ERR_ARG;
-  }
-  return cvchip_lime_write_spinor(filename, dims, s, 1, prec, checksum);
+  return cvchip::guarded("cvchip_lime_write_propagator", [&]() -> int {
+    if (!filename || !dims || !s) {
+      set_error("NULL argument to cvchip_lime_write_propagator");
+      return CVCHIP_ERR_NULL;
+    }
+    if (!check_dims(dims) || (prec != 64 && prec != 32)) {
+      set_error("invalid spinor format (precision %d)", prec);
+      return CVCHIP_ERR_ARG;
+    }
+    FILE *f = fopen(filename, "wb");
+    if (!f) {
+      set_error("could not open %s for writing", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    char msg[512];
+    snprintf(msg, sizeof(msg),
+             "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<etmcFormat>\n<field>diracFermion</field>\n<precision>%d"
+             "</precision>\n<flavours>%d</flavours>\n<lx>%d</lx>\n<ly>%d</ly>\n<lz>%d</lz>\n<lt>%d</lt>\n</etmcFormat>",
+             prec, 1, dims[1], dims[2], dims[3], dims[0]);
+    bool ok = write_record(f, "etmc-propagator-format", true, false, msg, strlen(msg));
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+      set_error("write error on %s", filename);
+      return CVCHIP_ERR_ARG;
+    }
+    return cvchip_lime_write_spinor(filename, dims, s, 1, prec, checksum);
+  });
 }

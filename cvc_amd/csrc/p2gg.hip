@@ -113,13 +113,15 @@ using namespace cvchip;
 
 extern "C" int cvchip_field_sequential_sources(cvchip_ctx *c, cvchip_field *out, const cvchip_field *in, int col0,
                                                int tseq, int gamma_id, const int mom[3], double sgn) {
-  if (!c || !out || !in) {
-    set_error("NULL argument to cvchip_field_sequential_sources");
-    return CVCHIP_ERR_NULL;
-  }
-  CVC_TRY(seq_sources(c, out->f, in->f, col0, tseq, gamma_id, mom, sgn));
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_sequential_sources", [&]() -> int {
+    if (!c || !out || !in) {
+      set_error("NULL argument to cvchip_field_sequential_sources");
+      return CVCHIP_ERR_NULL;
+    }
+    CVC_TRY(seq_sources(c, out->f, in->f, col0, tseq, gamma_id, mom, sgn));
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }
 
 // The whole P -> gamma gamma source point: 120 point propagators, 60
@@ -127,58 +129,60 @@ extern "C" int cvchip_field_sequential_sources(cvchip_ctx *c, cvchip_field *out,
 extern "C" int cvchip_p2gg_source(cvchip_ctx *c, double *conn, double contact[8], const int src[4], int seq_timeslice,
                                   int seq_gamma_id, const int seq_mom[3], int propagator_position, double sp,
                                   int niter_max, int prec, double delta, cvchip_stats *stats) {
-  if (!c || !conn || !contact || !src) {
-    set_error("NULL argument to cvchip_p2gg_source");
-    return CVCHIP_ERR_NULL;
-  }
-  if (propagator_position != 0 && propagator_position != 1) {
-    set_error("propagator_position must be 0 (up) or 1 (down), got %d", propagator_position);
-    return CVCHIP_ERR_ARG;
-  }
-  if (seq_gamma_id < 0 || seq_gamma_id > 15) {
-    set_error("sequential source gamma must be 0..15, got %d", seq_gamma_id);
-    return CVCHIP_ERR_ARG;
-  }
-  const int T = c->lat.T;
-  const int tseq = (((src[0] + seq_timeslice) % T) + T) % T;   // p2gg_xspace.c:446
-  const double sgn = 1. - 2. * propagator_position;
-  Field up, dn, seq, b, x;
-  int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, seq, 60, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, b, 12, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, x, 12, CVCHIP_PREC_F64);
-  cvchip_stats tot;
-  memset(&tot, 0, sizeof(tot));
-  if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
-  Field &P = propagator_position == 0 ? up : dn;
-  Field &Q = propagator_position == 0 ? dn : up;
-  const double mu0 = c->mu;
-  for (int s = 0; s < 5 && !rc; s++) {
-    rc = seq_sources(c, b, P, 12 * s, tseq, seq_gamma_id, seq_mom, sgn);
-    if (!rc && hipMemsetAsync(x.data, 0, x.bytes, c->stream) != hipSuccess) rc = CVCHIP_ERR_HIP;
-    if (rc) break;
-    c->mu = propagator_position == 0 ? mu0 : -mu0;
-    cvchip_stats st;
-    memset(&st, 0, sizeof(st));
-    rc = invert_her_fields(c, x, b, sp, niter_max, prec, delta, 0, &st);
-    c->mu = mu0;
-    if (rc > 0) rc = 0;
-    if (rc) break;
-    tot.iterations += st.iterations;
-    tot.converged += st.converged;
-    tot.reliable_updates += st.reliable_updates;
-    tot.dslash_calls += st.dslash_calls;
-    tot.seconds += st.seconds;
-    if (st.true_res2_max > tot.true_res2_max) tot.true_res2_max = st.true_res2_max;
-    rc = put_columns(c, seq, x, 12 * s);
-  }
-  if (!rc) rc = contract_fields(c, conn, contact, seq, Q, src);
-  if (stats) *stats = tot;
-  field_free(up);
-  field_free(dn);
-  field_free(seq);
-  field_free(b);
-  field_free(x);
-  return rc;
+  return cvchip::guarded("cvchip_p2gg_source", [&]() -> int {
+    if (!c || !conn || !contact || !src) {
+      set_error("NULL argument to cvchip_p2gg_source");
+      return CVCHIP_ERR_NULL;
+    }
+    if (propagator_position != 0 && propagator_position != 1) {
+      set_error("propagator_position must be 0 (up) or 1 (down), got %d", propagator_position);
+      return CVCHIP_ERR_ARG;
+    }
+    if (seq_gamma_id < 0 || seq_gamma_id > 15) {
+      set_error("sequential source gamma must be 0..15, got %d", seq_gamma_id);
+      return CVCHIP_ERR_ARG;
+    }
+    const int T = c->lat.T;
+    const int tseq = (((src[0] + seq_timeslice) % T) + T) % T;   // p2gg_xspace.c:446
+    const double sgn = 1. - 2. * propagator_position;
+    Field up, dn, seq, b, x;
+    int rc = field_alloc(c, up, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, dn, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, seq, 60, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, b, 12, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, x, 12, CVCHIP_PREC_F64);
+    cvchip_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    if (!rc) rc = solve_source_props(c, src, sp, niter_max, prec, delta, up, dn, &tot);
+    Field &P = propagator_position == 0 ? up : dn;
+    Field &Q = propagator_position == 0 ? dn : up;
+    const double mu0 = c->mu;
+    for (int s = 0; s < 5 && !rc; s++) {
+      rc = seq_sources(c, b, P, 12 * s, tseq, seq_gamma_id, seq_mom, sgn);
+      if (!rc && hipMemsetAsync(x.data, 0, x.bytes, c->stream) != hipSuccess) rc = CVCHIP_ERR_HIP;
+      if (rc) break;
+      c->mu = propagator_position == 0 ? mu0 : -mu0;
+      cvchip_stats st;
+      memset(&st, 0, sizeof(st));
+      rc = invert_her_fields(c, x, b, sp, niter_max, prec, delta, 0, &st);
+      c->mu = mu0;
+      if (rc > 0) rc = 0;
+      if (rc) break;
+      tot.iterations += st.iterations;
+      tot.converged += st.converged;
+      tot.reliable_updates += st.reliable_updates;
+      tot.dslash_calls += st.dslash_calls;
+      tot.seconds += st.seconds;
+      if (st.true_res2_max > tot.true_res2_max) tot.true_res2_max = st.true_res2_max;
+      rc = put_columns(c, seq, x, 12 * s);
+    }
+    if (!rc) rc = contract_fields(c, conn, contact, seq, Q, src);
+    if (stats) *stats = tot;
+    field_free(up);
+    field_free(dn);
+    field_free(seq);
+    field_free(b);
+    field_free(x);
+    return rc;
+  });
 }

@@ -1176,61 +1176,67 @@ using namespace cvchip;
 extern "C" int cvchip_field_invert(cvchip_ctx *c, cvchip_field *x, const cvchip_field *b, double sp,
                                    int niter_max, int prec, double delta, unsigned flags,
                                    cvchip_stats *stats) {
-  if (!c || !x || !b) {
-    set_error("NULL argument to cvchip_field_invert");
-    return CVCHIP_ERR_NULL;
-  }
-  return invert_fields(c, x->f, b->f, sp, niter_max, prec, delta, flags, stats);
+  return cvchip::guarded("cvchip_field_invert", [&]() -> int {
+    if (!c || !x || !b) {
+      set_error("NULL argument to cvchip_field_invert");
+      return CVCHIP_ERR_NULL;
+    }
+    return invert_fields(c, x->f, b->f, sp, niter_max, prec, delta, flags, stats);
+  });
 }
 
 extern "C" int cvchip_invert_Qtm(cvchip_ctx *c, double *x, const double *b, int nrhs, double sp,
                                  int niter_max, int prec, double delta, unsigned flags, cvchip_stats *stats) {
-  if (!c || !x || !b) {
-    set_error("NULL argument to cvchip_invert_Qtm");
-    return CVCHIP_ERR_NULL;
-  }
-  Field fx, fb;
-  int rc = field_alloc(c, fx, nrhs, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, fb, nrhs, CVCHIP_PREC_F64);
-  if (!rc) rc = upload_aos(c, fx, x, 0, nrhs);
-  if (!rc) rc = upload_aos(c, fb, b, 0, nrhs);
-  int ret = rc;
-  if (!rc) {
-    ret = invert_fields(c, fx, fb, sp, niter_max, prec, delta, flags, stats);
-    // the reference leaves x as computed also when not converged
-    if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
-      int rc2 = download_aos(c, fx, x, 0, nrhs);
-      if (rc2) ret = rc2;
+  return cvchip::guarded("cvchip_invert_Qtm", [&]() -> int {
+    if (!c || !x || !b) {
+      set_error("NULL argument to cvchip_invert_Qtm");
+      return CVCHIP_ERR_NULL;
     }
-  }
-  field_free(fx);
-  field_free(fb);
-  return ret;
+    Field fx, fb;
+    int rc = field_alloc(c, fx, nrhs, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, fb, nrhs, CVCHIP_PREC_F64);
+    if (!rc) rc = upload_aos(c, fx, x, 0, nrhs);
+    if (!rc) rc = upload_aos(c, fb, b, 0, nrhs);
+    int ret = rc;
+    if (!rc) {
+      ret = invert_fields(c, fx, fb, sp, niter_max, prec, delta, flags, stats);
+      // the reference leaves x as computed also when not converged
+      if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
+        int rc2 = download_aos(c, fx, x, 0, nrhs);
+        if (rc2) ret = rc2;
+      }
+    }
+    field_free(fx);
+    field_free(fb);
+    return ret;
+  });
 }
 
 extern "C" int cvchip_invert_Qtm_her(cvchip_ctx *c, double *x, const double *b, int nrhs, double sp,
                                      int niter_max, int prec, double delta, unsigned flags,
                                      cvchip_stats *stats) {
-  if (!c || !x || !b) {
-    set_error("NULL argument to cvchip_invert_Qtm_her");
-    return CVCHIP_ERR_NULL;
-  }
-  Field fx, fb;
-  int rc = field_alloc(c, fx, nrhs, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, fb, nrhs, CVCHIP_PREC_F64);
-  if (!rc) rc = upload_aos(c, fx, x, 0, nrhs);
-  if (!rc) rc = upload_aos(c, fb, b, 0, nrhs);
-  int ret = rc;
-  if (!rc) {
-    ret = invert_her_fields(c, fx, fb, sp, niter_max, prec, delta, flags, stats);
-    if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
-      int rc2 = download_aos(c, fx, x, 0, nrhs);
-      if (rc2) ret = rc2;
+  return cvchip::guarded("cvchip_invert_Qtm_her", [&]() -> int {
+    if (!c || !x || !b) {
+      set_error("NULL argument to cvchip_invert_Qtm_her");
+      return CVCHIP_ERR_NULL;
     }
-  }
-  field_free(fx);
-  field_free(fb);
-  return ret;
+    Field fx, fb;
+    int rc = field_alloc(c, fx, nrhs, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, fb, nrhs, CVCHIP_PREC_F64);
+    if (!rc) rc = upload_aos(c, fx, x, 0, nrhs);
+    if (!rc) rc = upload_aos(c, fb, b, 0, nrhs);
+    int ret = rc;
+    if (!rc) {
+      ret = invert_her_fields(c, fx, fb, sp, niter_max, prec, delta, flags, stats);
+      if (ret >= 0 || ret == CVCHIP_ERR_NOCONV) {
+        int rc2 = download_aos(c, fx, x, 0, nrhs);
+        if (rc2) ret = rc2;
+      }
+    }
+    field_free(fx);
+    field_free(fb);
+    return ret;
+  });
 }
 
 // spinor_scalar_product_co / _re (invert_Qtm.c:28-84) on host arrays
@@ -1260,63 +1266,71 @@ static int scalar_product(cvchip_ctx *c, double w[3], const double *xi, const do
 }
 
 extern "C" int cvchip_scalar_product_co(cvchip_ctx *c, double w[2], const double *xi, const double *phi) {
-  if (!c || !w || !xi || !phi) return CVCHIP_ERR_NULL;
-  double r[3];
-  CVC_TRY(scalar_product(c, r, xi, phi));
-  w[0] = r[0];
-  w[1] = r[1];
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_scalar_product_co", [&]() -> int {
+    if (!c || !w || !xi || !phi) return CVCHIP_ERR_NULL;
+    double r[3];
+    CVC_TRY(scalar_product(c, r, xi, phi));
+    w[0] = r[0];
+    w[1] = r[1];
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_scalar_product_re(cvchip_ctx *c, double *out, const double *xi, const double *phi) {
-  if (!c || !out || !xi || !phi) return CVCHIP_ERR_NULL;
-  double r[3];
-  CVC_TRY(scalar_product(c, r, xi, phi));
-  *out = r[0];
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_scalar_product_re", [&]() -> int {
+    if (!c || !out || !xi || !phi) return CVCHIP_ERR_NULL;
+    double r[3];
+    CVC_TRY(scalar_product(c, r, xi, phi));
+    *out = r[0];
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_invert_her(cvchip_ctx *c, cvchip_field *x, const cvchip_field *b, double sp,
                                        int niter_max, int prec, double delta, unsigned flags,
                                        cvchip_stats *stats) {
-  if (!c || !x || !b) {
-    set_error("NULL argument to cvchip_field_invert_her");
-    return CVCHIP_ERR_NULL;
-  }
-  if (x->f.prec != CVCHIP_PREC_F64 || b->f.prec != CVCHIP_PREC_F64 || x->f.nrhs != b->f.nrhs) {
-    set_error("solution and source must be FP64 sets of equal size");
-    return CVCHIP_ERR_ARG;
-  }
-  return invert_her_fields(c, x->f, b->f, sp, niter_max, prec, delta, flags, stats);
+  return cvchip::guarded("cvchip_field_invert_her", [&]() -> int {
+    if (!c || !x || !b) {
+      set_error("NULL argument to cvchip_field_invert_her");
+      return CVCHIP_ERR_NULL;
+    }
+    if (x->f.prec != CVCHIP_PREC_F64 || b->f.prec != CVCHIP_PREC_F64 || x->f.nrhs != b->f.nrhs) {
+      set_error("solution and source must be FP64 sets of equal size");
+      return CVCHIP_ERR_ARG;
+    }
+    return invert_her_fields(c, x->f, b->f, sp, niter_max, prec, delta, flags, stats);
+  });
 }
 
 extern "C" int cvchip_field_copy(cvchip_ctx *c, cvchip_field *dst, const cvchip_field *src) {
-  if (!c || !dst || !src) return CVCHIP_ERR_NULL;
-  Field &d = dst->f;
-  const Field &s = src->f;
-  if (d.nrhs != s.nrhs || d.V != s.V) {
-    set_error("field copy between sets of different size");
-    return CVCHIP_ERR_ARG;
-  }
-  const int N = s.nrhs;
-  const uint32_t VN = (uint32_t)(s.V * N);
-  const int B = block_for(N);
-  dim3 g((VN + B - 1) / B), b(B);
-  using I64 = IOPlain<double>;
-  using I32 = IOPlain<float>;
-#define CP(TD, TS) hipLaunchKernelGGL((k_copy<TD, TS>), g, b, 0, c->stream, mkio<TD>(d), mkio<TS>(s), \
-                                      (const RhsState *)nullptr, 0, VN, N)
-  if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F64) CP(I64, I64);
-  else if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F32) CP(I64, I32);
-  else if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F16) CP(I64, IOHalf);
-  else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F64) CP(I32, I64);
-  else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F32) CP(I32, I32);
-  else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F16) CP(I32, IOHalf);
-  else if (d.prec == CVCHIP_PREC_F16 && s.prec == CVCHIP_PREC_F64) CP(IOHalf, I64);
-  else if (d.prec == CVCHIP_PREC_F16 && s.prec == CVCHIP_PREC_F32) CP(IOHalf, I32);
-  else CP(IOHalf, IOHalf);
-#undef CP
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_copy", [&]() -> int {
+    if (!c || !dst || !src) return CVCHIP_ERR_NULL;
+    Field &d = dst->f;
+    const Field &s = src->f;
+    if (d.nrhs != s.nrhs || d.V != s.V) {
+      set_error("field copy between sets of different size");
+      return CVCHIP_ERR_ARG;
+    }
+    const int N = s.nrhs;
+    const uint32_t VN = (uint32_t)(s.V * N);
+    const int B = block_for(N);
+    dim3 g((VN + B - 1) / B), b(B);
+    using I64 = IOPlain<double>;
+    using I32 = IOPlain<float>;
+  #define CP(TD, TS) hipLaunchKernelGGL((k_copy<TD, TS>), g, b, 0, c->stream, mkio<TD>(d), mkio<TS>(s), \
+                                        (const RhsState *)nullptr, 0, VN, N)
+    if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F64) CP(I64, I64);
+    else if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F32) CP(I64, I32);
+    else if (d.prec == CVCHIP_PREC_F64 && s.prec == CVCHIP_PREC_F16) CP(I64, IOHalf);
+    else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F64) CP(I32, I64);
+    else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F32) CP(I32, I32);
+    else if (d.prec == CVCHIP_PREC_F32 && s.prec == CVCHIP_PREC_F16) CP(I32, IOHalf);
+    else if (d.prec == CVCHIP_PREC_F16 && s.prec == CVCHIP_PREC_F64) CP(IOHalf, I64);
+    else if (d.prec == CVCHIP_PREC_F16 && s.prec == CVCHIP_PREC_F32) CP(IOHalf, I32);
+    else CP(IOHalf, IOHalf);
+  #undef CP
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }

@@ -252,62 +252,68 @@ int twopoint_fields(cvchip_ctx *c, double *contr, int idsource, int idsink, cons
 using namespace cvchip;
 
 extern "C" int cvchip_gamma_tables(int perm[16][24], int sign[16][24]) {
-  if (!perm || !sign) return CVCHIP_ERR_NULL;
-  UMat g[16];
-  gamma_mats(g);
-  for (int n = 0; n < 16; n++)
-    for (int a = 0; a < 4; a++)
-      for (int col = 0; col < 3; col++) {
-        const int kr = 6 * a + 2 * col, kb = 6 * g[n].col[a] + 2 * col;
-        if (g[n].im[a] == 0) {   // +-1: re <- +-re, im <- +-im
-          perm[n][kr] = kb, sign[n][kr] = g[n].re[a];
-          perm[n][kr + 1] = kb + 1, sign[n][kr + 1] = g[n].re[a];
-        } else {                 // +-i: re <- -+im, im <- +-re
-          perm[n][kr] = kb + 1, sign[n][kr] = -g[n].im[a];
-          perm[n][kr + 1] = kb, sign[n][kr + 1] = g[n].im[a];
+  return cvchip::guarded("cvchip_gamma_tables", [&]() -> int {
+    if (!perm || !sign) return CVCHIP_ERR_NULL;
+    UMat g[16];
+    gamma_mats(g);
+    for (int n = 0; n < 16; n++)
+      for (int a = 0; a < 4; a++)
+        for (int col = 0; col < 3; col++) {
+          const int kr = 6 * a + 2 * col, kb = 6 * g[n].col[a] + 2 * col;
+          if (g[n].im[a] == 0) {   // +-1: re <- +-re, im <- +-im
+            perm[n][kr] = kb, sign[n][kr] = g[n].re[a];
+            perm[n][kr + 1] = kb + 1, sign[n][kr + 1] = g[n].re[a];
+          } else {                 // +-i: re <- -+im, im <- +-re
+            perm[n][kr] = kb + 1, sign[n][kr] = -g[n].im[a];
+            perm[n][kr + 1] = kb, sign[n][kr + 1] = g[n].im[a];
+          }
         }
-      }
-  return CVCHIP_OK;
+    return CVCHIP_OK;
+  });
 }
 
 extern "C" int cvchip_field_contract_twopoint(cvchip_ctx *c, double *contr, int idsource, int idsink,
                                               const cvchip_field *chi, const cvchip_field *phi, int n_c,
                                               const int snk_mom[3], const int src[4]) {
-  if (!c || !contr || !chi || !phi || !src) {
-    set_error("NULL argument to cvchip_field_contract_twopoint");
-    return CVCHIP_ERR_NULL;
-  }
-  return twopoint_fields(c, contr, idsource, idsink, chi->f, phi->f, n_c, snk_mom, src);
+  return cvchip::guarded("cvchip_field_contract_twopoint", [&]() -> int {
+    if (!c || !contr || !chi || !phi || !src) {
+      set_error("NULL argument to cvchip_field_contract_twopoint");
+      return CVCHIP_ERR_NULL;
+    }
+    return twopoint_fields(c, contr, idsource, idsink, chi->f, phi->f, n_c, snk_mom, src);
+  });
 }
 
 extern "C" int cvchip_contract_twopoint(cvchip_ctx *c, double *contr, int idsource, int idsink,
                                         const double *const *chi, const double *const *phi, int n_c,
                                         const int snk_mom[3], const int src[4]) {
-  if (!c || !contr || !chi || !phi || !src) {
-    set_error("NULL argument to cvchip_contract_twopoint");
-    return CVCHIP_ERR_NULL;
-  }
-  if (n_c < 1 || n_c > 3) {
-    set_error("n_c must be 1..3, got %d", n_c);
-    return CVCHIP_ERR_ARG;
-  }
-  Field fc, fp;
-  const int N = 4 * n_c;
-  int rc = field_alloc(c, fc, N, CVCHIP_PREC_F64);
-  if (!rc) rc = field_alloc(c, fp, N, CVCHIP_PREC_F64);
-  for (int k = 0; k < N && !rc; k++) {
-    if (!chi[k] || !phi[k]) {
-      set_error("NULL propagator column %d", k);
-      rc = CVCHIP_ERR_NULL;
-      break;
+  return cvchip::guarded("cvchip_contract_twopoint", [&]() -> int {
+    if (!c || !contr || !chi || !phi || !src) {
+      set_error("NULL argument to cvchip_contract_twopoint");
+      return CVCHIP_ERR_NULL;
     }
-    rc = upload_aos(c, fc, chi[k], k, 1);
-    if (!rc) rc = upload_aos(c, fp, phi[k], k, 1);
-  }
-  if (!rc) rc = twopoint_fields(c, contr, idsource, idsink, fc, fp, n_c, snk_mom, src);
-  field_free(fc);
-  field_free(fp);
-  return rc;
+    if (n_c < 1 || n_c > 3) {
+      set_error("n_c must be 1..3, got %d", n_c);
+      return CVCHIP_ERR_ARG;
+    }
+    Field fc, fp;
+    const int N = 4 * n_c;
+    int rc = field_alloc(c, fc, N, CVCHIP_PREC_F64);
+    if (!rc) rc = field_alloc(c, fp, N, CVCHIP_PREC_F64);
+    for (int k = 0; k < N && !rc; k++) {
+      if (!chi[k] || !phi[k]) {
+        set_error("NULL propagator column %d", k);
+        rc = CVCHIP_ERR_NULL;
+        break;
+      }
+      rc = upload_aos(c, fc, chi[k], k, 1);
+      if (!rc) rc = upload_aos(c, fp, phi[k], k, 1);
+    }
+    if (!rc) rc = twopoint_fields(c, contr, idsource, idsink, fc, fp, n_c, snk_mom, src);
+    field_free(fc);
+    field_free(fp);
+    return rc;
+  });
 }
 
 // No counterpart in the reference.  cvchip_field_contract_twopoint for callers
@@ -319,30 +325,32 @@ extern "C" int cvchip_contract_twopoint(cvchip_ctx *c, double *contr, int idsour
 extern "C" int cvchip_field_contract_twopoint_noalloc(cvchip_ctx *c, double *contr_dev, int idsource, int idsink,
                                                       const cvchip_field *chi, const cvchip_field *phi, int n_c,
                                                       const int snk_mom[3], const int src[4]) {
-  if (!c || !contr_dev || !chi || !phi || !src) {
-    set_error("NULL argument to cvchip_field_contract_twopoint_noalloc");
-    return CVCHIP_ERR_NULL;
-  }
-  const Lattice &L = c->lat;
-  const size_t need = 2 * (size_t)L.T * (size_t)((L.LX * L.LY * L.LZ + 255) / 256);
-  if (need > kTwoPtScratch) {
-    set_error("two-point function without allocation: %zu partial sums exceed the scratch of %zu "
-              "(T * ceil(LX LY LZ / 256) must stay at or below %zu)", need, kTwoPtScratch, kTwoPtScratch / 2);
-    return CVCHIP_ERR_ARG;
-  }
-  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) {
-    set_error("two-point scratch of device %d not set up (create the context on that device)", c->device);
-    return CVCHIP_ERR_STATE;
-  }
-  double *part = g_scratch_ptr[c->device];
-  TwoPt P;
-  size_t nblk = 0;
-  CVC_TRY(twopoint_params(c, idsource, idsink, chi->f, phi->f, n_c, snk_mom, src, P, nblk));
-  hipLaunchKernelGGL(k_twopoint, dim3((unsigned)nblk), dim3(256), 0, c->stream, (const double2 *)chi->f.data,
-                     (const double2 *)phi->f.data, P, part);
-  hipLaunchKernelGGL(k_twopoint_final, dim3(1), dim3(256), 0, c->stream, (const double *)part, contr_dev, L.T,
-                     P.nchunk);
-  CVC_HIP(hipGetLastError());
-  CVC_HIP(hipStreamSynchronize(c->stream));
-  return CVCHIP_OK;
+  return cvchip::guarded("cvchip_field_contract_twopoint_noalloc", [&]() -> int {
+    if (!c || !contr_dev || !chi || !phi || !src) {
+      set_error("NULL argument to cvchip_field_contract_twopoint_noalloc");
+      return CVCHIP_ERR_NULL;
+    }
+    const Lattice &L = c->lat;
+    const size_t need = 2 * (size_t)L.T * (size_t)((L.LX * L.LY * L.LZ + 255) / 256);
+    if (need > kTwoPtScratch) {
+      set_error("two-point function without allocation: %zu partial sums exceed the scratch of %zu "
+                "(T * ceil(LX LY LZ / 256) must stay at or below %zu)", need, kTwoPtScratch, kTwoPtScratch / 2);
+      return CVCHIP_ERR_ARG;
+    }
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) {
+      set_error("two-point scratch of device %d not set up (create the context on that device)", c->device);
+      return CVCHIP_ERR_STATE;
+    }
+    double *part = g_scratch_ptr[c->device];
+    TwoPt P;
+    size_t nblk = 0;
+    CVC_TRY(twopoint_params(c, idsource, idsink, chi->f, phi->f, n_c, snk_mom, src, P, nblk));
+    hipLaunchKernelGGL(k_twopoint, dim3((unsigned)nblk), dim3(256), 0, c->stream, (const double2 *)chi->f.data,
+                       (const double2 *)phi->f.data, P, part);
+    hipLaunchKernelGGL(k_twopoint_final, dim3(1), dim3(256), 0, c->stream, (const double *)part, contr_dev, L.T,
+                       P.nchunk);
+    CVC_HIP(hipGetLastError());
+    CVC_HIP(hipStreamSynchronize(c->stream));
+    return CVCHIP_OK;
+  });
 }

@@ -214,3 +214,19 @@ def test_spinor_errors(tmp_path):
     path.write_bytes(bytes(raw))
     with pytest.raises(cvc_amd.CvchipError):
         cvc_amd.lime_read_spinor(path, dims, 0)
+
+
+@pytest.mark.parametrize("length", [2 ** 64 - 64, 2 ** 63, 10 ** 6])
+def test_malformed_record_length_is_rejected(tmp_path, length):
+    """A record header whose length runs past the end of the file (also one
+    near 2^64, where offset + length wraps around) makes the reader return an
+    error instead of allocating that length."""
+    dims = (2, 2, 2, 2)
+    path = tmp_path / "c"
+    cvc_amd.lime_write_contraction(path, np.ones((16, 16, 2)), dims)
+    raw = bytearray(path.read_bytes())
+    hdr = raw.find(b"scidac-binary-data") - 16
+    raw[hdr + 8:hdr + 16] = struct.pack(">Q", length)
+    path.write_bytes(bytes(raw))
+    with pytest.raises(cvc_amd.CvchipError):
+        cvc_amd.lime_read_contraction(path, dims)
